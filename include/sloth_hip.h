@@ -1,0 +1,203 @@
+/*
+ * sloth_hip.h -- C-ABI of the MI355X rasterization library (libsloth_hip.so).
+ *
+ * Drop-in boundary for the hot path of ecumene/rust-sloth: the reference's
+ * `Context` (src/context.rs:12-19), `SimpleMesh` (src/geometry.rs:78-81) and
+ * `draw_mesh` (src/rasterizer.rs:39-46).  Plain pointers and sizes only; every
+ * entry point returns an int status (SLOTH_OK = 0) and never throws or aborts
+ * across the boundary.  sloth_last_error() describes the last failure on the
+ * calling thread.
+ *
+ * Data formats
+ *   positions  float[n][3][3]  triangle soup, vertex v1,v2,v3 as x,y,z; w = 1
+ *              (every constructor in the reference sets w = 1:
+ *              geometry.rs:26-32, 93-98, 166-170)
+ *   colours    uint8[n][3]     the per-triangle `color` (geometry.rs:18)
+ *   matrices   float[16]       4x4, column-major (nalgebra's storage order)
+ *   cells      uint32[W*H (+H in image mode)]  byte0 = glyph, byte1..3 = R,G,B;
+ *              the (char, (u8,u8,u8)) of Context.frame_buffer (context.rs:16)
+ *   z          float[W*H]      Context.z_buffer (context.rs:17)
+ *   winners    int32[W*H]      frame-global index of the triangle that owns
+ *              each z slot, -1 if none (not in the reference; for tests)
+ *
+ * Threading: a handle must not be used from two threads at once.  Different
+ * handles are independent.  All device work of a handle runs on one HIP
+ * stream; functions that return host data synchronise it.
+ */
+#ifndef SLOTH_HIP_H
+#define SLOTH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLOTH_OK 0
+#define SLOTH_EINVAL 1     /* bad argument (null handle, W or H outside 1..65535, ...) */
+#define SLOTH_ENOMEM 2     /* device or host allocation failed */
+#define SLOTH_EHIP 3       /* HIP runtime error (message in sloth_last_error) */
+#define SLOTH_ENODEV 4     /* no usable gfx950 device */
+#define SLOTH_ESTATE 5     /* call not valid in the handle's current state */
+
+#define SLOTH_MAX_RAMP 16
+
+/* A step shader.  The reference passes a closure `Fn(f32) -> char`
+ * (rasterizer.rs:39-42); a closure cannot cross the ABI, so the shader is a
+ * table: the first i with shade <= thresholds[i] gives glyphs[i], otherwise
+ * `fallback` (NaN included).  default_shader (rasterizer.rs:5-27) is
+ * n = 9, thresholds {0.2,...,1.0}, glyphs ".:-=+*#%@", fallback ' '.
+ * Glyphs are 8-bit characters. */
+typedef struct sloth_ramp {
+    uint32_t n;
+    float thresholds[SLOTH_MAX_RAMP];
+    uint8_t glyphs[SLOTH_MAX_RAMP];
+    uint8_t fallback;
+    uint8_t reserved[3];
+} sloth_ramp;
+
+typedef struct sloth_mesh sloth_mesh;
+typedef struct sloth_context sloth_context;
+typedef struct sloth_batch sloth_batch;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* Selects the HIP device for handles created afterwards (default 0) and
+ * checks that it is a gfx950 part.  No reference counterpart (the reference
+ * has no device). */
+int sloth_init(int device);
+
+/* Thread-local description of the last failed call ("" if none). */
+const char *sloth_last_error(void);
+
+/* Fills `out` with default_shader's table (rasterizer.rs:5-27). */
+int sloth_default_ramp(sloth_ramp *out);
+
+/* ---- SimpleMesh (geometry.rs:78-81) ----------------------------------- */
+
+/* Uploads a triangle soup to device memory.  n may be 0.  The bounding box
+ * follows the OBJ loader (geometry.rs:85-88,129-135): it starts at the origin
+ * and grows by every triangle's aabb. */
+int sloth_mesh_create(const float *positions, const uint8_t *colors, uint32_t n, sloth_mesh **out);
+
+/* Wraps positions/colours that already live in device memory (no copy; the
+ * caller keeps them alive and unchanged while the mesh is used).  The
+ * bounding box is computed on the device. */
+int sloth_mesh_create_device(const float *d_positions, const uint8_t *d_colors, uint32_t n,
+                             sloth_mesh **out);
+
+/* SimpleMesh.bounding_box (geometry.rs:79): min[3], max[3]. */
+int sloth_mesh_bbox(const sloth_mesh *mesh, float *min3, float *max3);
+uint32_t sloth_mesh_triangles(const sloth_mesh *mesh);
+int sloth_mesh_destroy(sloth_mesh *mesh);
+
+/* ---- Context (context.rs:12-142) -------------------------------------- */
+
+/* Context::blank (context.rs:22-34) followed by a fixed size, as image mode's
+ * match_dimensions (inputs.rs:159-169) sets it.  width, height in 1..65535
+ * (Context::update passes them through u16, context.rs:99).  image != 0 adds
+ * one newline cell per row (context.rs:38-40).  utransform starts as the
+ * identity. */
+int sloth_context_create(uint32_t width, uint32_t height, int image, sloth_context **out);
+int sloth_context_destroy(sloth_context *ctx);
+
+/* Context::update (context.rs:93-141): utransform from the size and the
+ * largest bounding-box maximum of the given meshes.  bbox_max: [n][3]. */
+int sloth_context_update(sloth_context *ctx, const float *bbox_max, uint32_t n_meshes);
+
+/* Context.utransform (context.rs:13), column-major. */
+int sloth_context_set_utransform(sloth_context *ctx, const float *m16);
+int sloth_context_get_utransform(const sloth_context *ctx, float *m16);
+
+/* Context::clear (context.rs:35-45): all cells (' ', (0,0,0)), z = f32::MAX,
+ * and the list of draws since the last clear is emptied. */
+int sloth_context_clear(sloth_context *ctx);
+
+/* rasterizer.rs:39-46 draw_mesh(context, mesh, transform, shader).  Records
+ * the draw; the frame is rasterized when its buffers are read (or by
+ * sloth_context_render).  Triangle order across draws since the last clear is
+ * the z-tie order, as in the reference.  The mesh must stay alive until the
+ * next clear.  shader = NULL means default_shader. */
+int sloth_draw_mesh(sloth_context *ctx, const sloth_mesh *mesh, const float *transform16,
+                    const sloth_ramp *shader);
+
+/* Rasterizes every draw recorded since the last clear into the context's
+ * device buffers, on `stream` (a hipStream_t; NULL = the context's own
+ * stream).  Returns without waiting for the device. */
+int sloth_context_render(sloth_context *ctx, void *stream);
+
+/* Renders if needed, then copies the frame buffer, z buffer and winning
+ * triangle ids to the host (any pointer may be NULL) and waits. */
+int sloth_context_read(sloth_context *ctx, uint32_t *cells, float *z, int32_t *winners);
+
+/* Device addresses of the context's cell and z buffers (valid until the
+ * context is destroyed), and their element counts. */
+int sloth_context_device_buffers(sloth_context *ctx, uint32_t **d_cells, float **d_z, uint64_t *n_cells,
+                                 uint64_t *n_z);
+
+/* ---- batches (many frames in one launch sequence; no reference
+ * counterpart: the reference renders one frame per loop iteration,
+ * main.rs:60-111) ---------------------------------------------------- */
+
+typedef struct sloth_frame_desc {
+    uint32_t width, height;   /* 1..65535 */
+    int32_t image;
+    float utransform[16];     /* column-major; e.g. from sloth_screen_transform */
+} sloth_frame_desc;
+
+typedef struct sloth_draw_desc {
+    uint32_t frame;           /* index into the frame array */
+    const sloth_mesh *mesh;
+    float transform[16];      /* column-major */
+} sloth_draw_desc;
+
+/* Builds a batch: every frame cleared, then its draws in array order (draws
+ * of one frame need not be contiguous; their relative order is the frame's
+ * draw order).  Output buffers for all frames are allocated in device memory.
+ * shader = NULL means default_shader. */
+int sloth_batch_create(const sloth_frame_desc *frames, uint32_t n_frames, const sloth_draw_desc *draws,
+                       uint32_t n_draws, const sloth_ramp *shader, sloth_batch **out);
+
+/* Replaces the per-draw transforms (float[n_draws][16]) without rebuilding. */
+int sloth_batch_set_transforms(sloth_batch *b, const float *transforms16);
+
+/* Renders all frames on `stream` (NULL = the batch's own stream).  The first
+ * call on a stream captures the launch sequence into a HIP graph; later calls
+ * replay it.  Returns without waiting. */
+int sloth_batch_run(sloth_batch *b, void *stream);
+
+/* Renders every frame with one launch sequence per frame and a device
+ * synchronisation after each (the plain, unbatched form).  Waits. */
+int sloth_batch_run_per_frame(sloth_batch *b, void *stream);
+
+/* Waits for the batch's stream. */
+int sloth_batch_sync(sloth_batch *b);
+
+/* Copies one frame's cells / z / winners to the host (NULL = skip); waits. */
+int sloth_batch_read_frame(sloth_batch *b, uint32_t frame, uint32_t *cells, float *z, int32_t *winners);
+
+/* Device addresses of one frame's cells and z. */
+int sloth_batch_device_frame(sloth_batch *b, uint32_t frame, uint32_t **d_cells, float **d_z);
+
+/* Times each stage of one (non-graph) run with HIP events on the batch's
+ * stream: out_ms[i] = milliseconds of stage i, names[i] its kernel name;
+ * *n_stages in: capacity, out: count. */
+int sloth_batch_profile(sloth_batch *b, float *out_ms, const char **names, uint32_t *n_stages);
+
+int sloth_batch_destroy(sloth_batch *b);
+
+/* ---- helpers ---------------------------------------------------------- */
+
+/* Context::update's utransform for a size and bounding-box maxima
+ * (context.rs:104-133), without a context. */
+int sloth_screen_transform(uint32_t width, uint32_t height, const float *bbox_max, uint32_t n_meshes,
+                           float *out16);
+
+/* utransform * transform in nalgebra's Matrix4 product order
+ * (rasterizer.rs:57). */
+int sloth_mat4_mul(const float *a16, const float *b16, float *out16);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SLOTH_HIP_H */

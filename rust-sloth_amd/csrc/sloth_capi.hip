@@ -1,0 +1,894 @@
+// Host side of libsloth_hip.so: the C-ABI declared in include/sloth_hip.h.
+//
+// A "job" is a set of frames (Contexts) and their draw_mesh calls, laid out in
+// device tables (Frame, Draw, Ramp) plus the output buffers of all frames.
+// A context is a job with one frame; a batch is a job with many, whose launch
+// sequence is captured once into a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sloth_hip.h"
+#include "sloth_kernels.h"
+#include "sloth_tiled.h"
+
+using namespace sloth;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+static int g_device = 0;
+
+static int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(SLOTH_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define ARG_CHECK(cond, msg) \
+    do {                     \
+        if (!(cond)) return fail(SLOTH_EINVAL, "%s", msg); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// host arithmetic in the reference's order (compiled with -ffp-contract=off)
+// ---------------------------------------------------------------------------
+
+// Matrix4 * Matrix4 as nalgebra's gemm computes it (rasterizer.rs:57): each
+// result column is a gemv, i.e. an axpy over the columns of the left operand.
+static void mat4_mul(const float* A, const float* B, float* out)
+{
+    float r[16];
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) {
+            float acc = A[0 * 4 + i] * B[j * 4 + 0];
+            acc = A[1 * 4 + i] * B[j * 4 + 1] + acc;
+            acc = A[2 * 4 + i] * B[j * 4 + 2] + acc;
+            acc = A[3 * 4 + i] * B[j * 4 + 3] + acc;
+            r[j * 4 + i] = acc;
+        }
+    memcpy(out, r, sizeof r);
+}
+
+// Context::update (context.rs:104-133).
+static void screen_transform(uint32_t W, uint32_t H, const float* bbox_max, uint32_t n, float* U)
+{
+    float scale = 0.0f;
+    for (uint32_t m = 0; m < n; ++m) {
+        scale = fmaxf(scale, bbox_max[3 * m + 0]);
+        scale = fmaxf(scale, bbox_max[3 * m + 1]);
+        scale = fmaxf(scale, bbox_max[3 * m + 2]);
+    }
+    float fw = (float)(uint16_t)W, fh = (float)(uint16_t)H;
+    scale = fminf(fh, fw / 2.0f) / scale / 2.0f;
+    float M[16] = {scale, 0.0f, 0.0f, 0.0f, 0.0f, -scale, 0.0f, 0.0f,
+                   0.0f,  0.0f, scale, 0.0f, fw / 4.0f, fh / 2.0f, 0.0f, 1.0f};
+    memcpy(U, M, sizeof M);
+}
+
+static const float IDENTITY[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+
+static Ramp default_ramp()
+{
+    static const float t[9] = {0.20f, 0.30f, 0.40f, 0.50f, 0.60f, 0.70f, 0.80f, 0.90f, 1.0f};
+    static const char g[9] = {'.', ':', '-', '=', '+', '*', '#', '%', '@'};
+    Ramp r;
+    memset(&r, 0, sizeof r);
+    r.n = 9;
+    for (int i = 0; i < 9; ++i) {
+        r.thr[i] = t[i];
+        r.glyph[i] = (uint8_t)g[i];
+    }
+    r.fallback = ' ';
+    return r;
+}
+
+static int to_ramp(const sloth_ramp* s, Ramp& r)
+{
+    if (!s) {
+        r = default_ramp();
+        return SLOTH_OK;
+    }
+    ARG_CHECK(s->n <= SLOTH_MAX_RAMP, "shader table has more than 16 steps");
+    memset(&r, 0, sizeof r);
+    r.n = s->n;
+    for (uint32_t i = 0; i < s->n; ++i) {
+        r.thr[i] = s->thresholds[i];
+        r.glyph[i] = s->glyphs[i];
+    }
+    r.fallback = s->fallback;
+    return SLOTH_OK;
+}
+
+static bool same_ramp(const Ramp& a, const Ramp& b)
+{
+    if (a.n != b.n || a.fallback != b.fallback) return false;
+    for (uint32_t i = 0; i < a.n; ++i)
+        if (memcmp(&a.thr[i], &b.thr[i], 4) != 0 || a.glyph[i] != b.glyph[i]) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grow-only allocation; contents are not preserved.
+    int reserve(size_t n)
+    {
+        if (n <= bytes && p) return SLOTH_OK;
+        release();
+        if (n == 0) return SLOTH_OK;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(SLOTH_ENOMEM, "hipMalloc(%zu bytes): %s", n, hipGetErrorString(e));
+        }
+        bytes = n;
+        return SLOTH_OK;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+// ---------------------------------------------------------------------------
+// meshes
+// ---------------------------------------------------------------------------
+struct sloth_mesh {
+    const float* d_xyz = nullptr;
+    const uint8_t* d_rgb = nullptr;
+    uint32_t n = 0;
+    DevBuf own_xyz, own_rgb;
+    float bbmin[3] = {0, 0, 0}, bbmax[3] = {0, 0, 0};
+};
+
+// OBJ-style bounding box (geometry.rs:85-88, 129-135).
+static void host_bbox(const float* xyz, uint32_t n, float* mn, float* mx)
+{
+    for (int c = 0; c < 3; ++c) mn[c] = mx[c] = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* t = xyz + (size_t)i * 9;
+        for (int c = 0; c < 3; ++c) {
+            float lo = fminf(t[c], fminf(t[3 + c], t[6 + c]));
+            float hi = fmaxf(t[c], fmaxf(t[3 + c], t[6 + c]));
+            mn[c] = fminf(lo, mn[c]);
+            mx[c] = fmaxf(hi, mx[c]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// jobs
+// ---------------------------------------------------------------------------
+struct DrawIn {
+    uint32_t frame;
+    const sloth_mesh* mesh;
+    float M[16];
+    uint32_t ramp;
+};
+
+struct FrameIn {
+    uint32_t W, H;
+    int image;
+};
+
+enum PathKind { PATH_GLOBAL = 0, PATH_TILED = 1 };
+
+struct Job {
+    std::vector<FrameIn> fin;
+    std::vector<DrawIn> din;
+    std::vector<Ramp> ramps;
+
+    // laid-out tables
+    std::vector<Frame> frames;
+    std::vector<Draw> draws;
+    uint64_t n_cells = 0, n_z = 0, n_rows = 0, n_items = 0;
+
+    DevBuf d_frames, d_draws, d_ramps, d_cells, d_z, d_keys, d_nl, d_win;
+    TiledPlan tiled;
+    int path = PATH_GLOBAL;
+    bool laid_out = false;
+
+    uint32_t add_ramp(const Ramp& r)
+    {
+        for (size_t i = 0; i < ramps.size(); ++i)
+            if (same_ramp(ramps[i], r)) return (uint32_t)i;
+        ramps.push_back(r);
+        return (uint32_t)(ramps.size() - 1);
+    }
+
+    // Lays out frames and draws (draws grouped by frame, order within a frame
+    // kept) and allocates device tables and outputs.
+    int layout()
+    {
+        const uint32_t F = (uint32_t)fin.size();
+        std::vector<uint32_t> order(din.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (uint32_t)i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return din[a].frame < din[b].frame; });
+        frames.assign(F, Frame{});
+        draws.assign(din.size(), Draw{});
+        uint64_t cells = 0, z = 0, rows = 0, items = 0;
+        size_t k = 0;
+        bool all_even = true;
+        for (uint32_t f = 0; f < F; ++f) {
+            Frame& fr = frames[f];
+            fr.W = fin[f].W;
+            fr.H = fin[f].H;
+            fr.image = fin[f].image ? 1u : 0u;
+            fr.cell_off = cells;
+            fr.cell_begin = cells;
+            fr.z_off = z;
+            fr.nl_off = rows;
+            fr.draw_begin = (uint32_t)k;
+            uint32_t tri = 0;
+            while (k < order.size() && din[order[k]].frame == f) {
+                const DrawIn& di = din[order[k]];
+                Draw& d = draws[k];
+                d.xyz = di.mesh->d_xyz;
+                d.rgb = di.mesh->d_rgb;
+                d.n = di.mesh->n;
+                d.tri_begin = tri;
+                d.frame = f;
+                d.ramp = di.ramp;
+                d.item_begin = items;
+                memcpy(d.M, di.M, sizeof d.M);
+                if ((uint64_t)tri + d.n >= (1ull << 31))
+                    return fail(SLOTH_EINVAL, "frame %u has 2^31 or more triangles", f);
+                tri += d.n;
+                items += d.n;
+                ++k;
+            }
+            fr.draw_end = (uint32_t)k;
+            fr.n_tris = tri;
+            uint64_t WH = (uint64_t)fr.W * fr.H;
+            cells += WH + (fr.image ? fr.H : 0);
+            z += WH;
+            rows += fr.H;
+            if (fr.W & 1u) all_even = false;
+        }
+        n_cells = cells;
+        n_z = z;
+        n_rows = rows;
+        n_items = items;
+        path = PATH_GLOBAL;
+        (void)all_even;
+        int rc;
+        if ((rc = d_frames.reserve(sizeof(Frame) * std::max<size_t>(1, frames.size())))) return rc;
+        if ((rc = d_draws.reserve(sizeof(Draw) * std::max<size_t>(1, draws.size())))) return rc;
+        if ((rc = d_ramps.reserve(sizeof(Ramp) * std::max<size_t>(1, ramps.size())))) return rc;
+        if ((rc = d_cells.reserve(sizeof(uint32_t) * n_cells))) return rc;
+        if ((rc = d_z.reserve(sizeof(float) * n_z))) return rc;
+        if (path == PATH_GLOBAL) {
+            if ((rc = d_keys.reserve(sizeof(uint64_t) * n_z))) return rc;
+            if ((rc = d_nl.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_rows)))) return rc;
+        } else {
+            if ((rc = tiled.plan(frames, draws, n_rows))) return rc;
+        }
+        laid_out = true;
+        return SLOTH_OK;
+    }
+
+    static bool force_global()
+    {
+        const char* e = getenv("SLOTH_FORCE_GLOBAL");
+        return e && e[0] == '1';
+    }
+
+    int upload(hipStream_t s)
+    {
+        if (!frames.empty())
+            HIP_TRY(hipMemcpyAsync(d_frames.p, frames.data(), sizeof(Frame) * frames.size(), hipMemcpyHostToDevice, s));
+        if (!draws.empty())
+            HIP_TRY(hipMemcpyAsync(d_draws.p, draws.data(), sizeof(Draw) * draws.size(), hipMemcpyHostToDevice, s));
+        if (!ramps.empty())
+            HIP_TRY(hipMemcpyAsync(d_ramps.p, ramps.data(), sizeof(Ramp) * ramps.size(), hipMemcpyHostToDevice, s));
+        if (path == PATH_TILED) {
+            int rc = tiled.upload(s);
+            if (rc) return rc;
+        }
+        // the host vectors may change after return: wait for the copies
+        HIP_TRY(hipStreamSynchronize(s));
+        return SLOTH_OK;
+    }
+
+    static unsigned blocks_for(uint64_t n, unsigned per = 256)
+    {
+        return (unsigned)((n + per - 1) / per);
+    }
+
+    // Global path over frames [f_lo, f_hi).
+    int launch_global(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, std::vector<hipEvent_t>* ev)
+    {
+        const Frame& a = frames[f_lo];
+        const Frame& b = frames[f_hi - 1];
+        uint64_t z_lo = a.z_off, z_hi = b.z_off + (uint64_t)b.W * b.H;
+        uint64_t r_lo = a.nl_off, r_hi = b.nl_off + b.H;
+        uint32_t d_lo = a.draw_begin, d_hi = b.draw_end;
+        uint64_t it_lo = d_lo < draws.size() ? draws[d_lo].item_begin : n_items;
+        uint64_t it_hi = d_hi < draws.size() ? draws[d_hi].item_begin : n_items;
+        uint64_t c_lo = a.cell_begin;
+        uint64_t c_hi = b.cell_begin + (uint64_t)b.W * b.H + (b.image ? b.H : 0);
+        if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
+        {
+            uint64_t n = std::max(z_hi - z_lo, r_hi - r_lo);
+            unsigned g = std::min<unsigned>(blocks_for(n), 8192u);
+            hipLaunchKernelGGL(k_clear_global, dim3(std::max(1u, g)), dim3(256), 0, s, d_keys.as<uint64_t>() + z_lo,
+                               z_hi - z_lo, d_nl.as<int32_t>() + r_lo, r_hi - r_lo);
+            HIP_TRY(hipGetLastError());
+        }
+        if (ev) HIP_TRY(hipEventRecord((*ev)[1], s));
+        if (it_hi > it_lo) {
+            hipLaunchKernelGGL(k_raster_global, dim3(blocks_for(it_hi - it_lo)), dim3(256), 0, s, d_draws.as<Draw>(),
+                               d_lo, d_hi, d_frames.as<Frame>(), it_lo, it_hi - it_lo, d_keys.as<uint64_t>(),
+                               d_nl.as<int32_t>());
+            HIP_TRY(hipGetLastError());
+        }
+        if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
+        if (c_hi > c_lo) {
+            hipLaunchKernelGGL(k_resolve_global, dim3(blocks_for(c_hi - c_lo)), dim3(256), 0, s, d_draws.as<Draw>(),
+                               d_ramps.as<Ramp>(), d_frames.as<Frame>(), f_lo, f_hi, c_lo, c_hi - c_lo,
+                               d_keys.as<uint64_t>(), d_nl.as<int32_t>(), d_cells.as<uint32_t>(), d_z.as<float>(), win);
+            HIP_TRY(hipGetLastError());
+        }
+        if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
+        return SLOTH_OK;
+    }
+
+    int launch(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, std::vector<hipEvent_t>* ev = nullptr)
+    {
+        if (f_lo >= f_hi) return SLOTH_OK;
+        if (path == PATH_TILED)
+            return tiled.launch(frames, draws, d_frames.as<Frame>(), d_draws.as<Draw>(), d_ramps.as<Ramp>(), f_lo,
+                                f_hi, d_cells.as<uint32_t>(), d_z.as<float>(), win, s, ev);
+        return launch_global(f_lo, f_hi, s, win, ev);
+    }
+
+    int ensure_win()
+    {
+        return d_win.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_z));
+    }
+};
+
+static hipStream_t make_stream()
+{
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// contexts
+// ---------------------------------------------------------------------------
+struct sloth_context {
+    Job job;
+    float U[16];
+    hipStream_t stream = nullptr;
+    bool rendered = false;      // device buffers match the draw list
+    bool win_valid = false;     // d_win matches the draw list
+    ~sloth_context()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+struct sloth_batch {
+    Job job;
+    std::vector<float> U;       // per frame utransform
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool ran = false;
+    std::vector<bool> win_valid;
+    ~sloth_batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* sloth_last_error(void) { return g_err.c_str(); }
+
+int sloth_init(int device)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return fail(SLOTH_ENODEV, "no HIP device (%s)", hipGetErrorString(e));
+    }
+    ARG_CHECK(device >= 0 && device < n, "device index out of range");
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, device));
+    if (strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        return fail(SLOTH_ENODEV, "device %d is %s, this library is built for gfx950", device, p.gcnArchName);
+    HIP_TRY(hipSetDevice(device));
+    g_device = device;
+    return SLOTH_OK;
+}
+
+static int ensure_device()
+{
+    static thread_local bool ok = false;
+    if (ok) return SLOTH_OK;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return fail(SLOTH_ENODEV, "no HIP device (%s)", hipGetErrorString(e));
+    }
+    HIP_TRY(hipSetDevice(g_device));
+    ok = true;
+    return SLOTH_OK;
+}
+
+int sloth_default_ramp(sloth_ramp* out)
+{
+    ARG_CHECK(out, "null output");
+    Ramp r = default_ramp();
+    memset(out, 0, sizeof *out);
+    out->n = r.n;
+    for (uint32_t i = 0; i < r.n; ++i) {
+        out->thresholds[i] = r.thr[i];
+        out->glyphs[i] = r.glyph[i];
+    }
+    out->fallback = r.fallback;
+    return SLOTH_OK;
+}
+
+int sloth_screen_transform(uint32_t width, uint32_t height, const float* bbox_max, uint32_t n_meshes, float* out16)
+{
+    ARG_CHECK(out16 && (bbox_max || n_meshes == 0), "null argument");
+    screen_transform(width, height, bbox_max, n_meshes, out16);
+    return SLOTH_OK;
+}
+
+int sloth_mat4_mul(const float* a16, const float* b16, float* out16)
+{
+    ARG_CHECK(a16 && b16 && out16, "null argument");
+    mat4_mul(a16, b16, out16);
+    return SLOTH_OK;
+}
+
+// ---- meshes ---------------------------------------------------------------
+
+int sloth_mesh_create(const float* positions, const uint8_t* colors, uint32_t n, sloth_mesh** out)
+{
+    ARG_CHECK(out, "null output");
+    *out = nullptr;
+    ARG_CHECK(n == 0 || (positions && colors), "null positions or colours");
+    int rc = ensure_device();
+    if (rc) return rc;
+    sloth_mesh* m = new (std::nothrow) sloth_mesh();
+    if (!m) return fail(SLOTH_ENOMEM, "host allocation failed");
+    m->n = n;
+    host_bbox(positions, n, m->bbmin, m->bbmax);
+    if (n) {
+        if ((rc = m->own_xyz.reserve((size_t)n * 36)) || (rc = m->own_rgb.reserve((size_t)n * 3))) {
+            delete m;
+            return rc;
+        }
+        hipError_t e1 = hipMemcpy(m->own_xyz.p, positions, (size_t)n * 36, hipMemcpyHostToDevice);
+        hipError_t e2 = hipMemcpy(m->own_rgb.p, colors, (size_t)n * 3, hipMemcpyHostToDevice);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            delete m;
+            return fail(SLOTH_EHIP, "mesh upload: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        }
+        m->d_xyz = m->own_xyz.as<float>();
+        m->d_rgb = m->own_rgb.as<uint8_t>();
+    }
+    *out = m;
+    return SLOTH_OK;
+}
+
+int sloth_mesh_create_device(const float* d_positions, const uint8_t* d_colors, uint32_t n, sloth_mesh** out)
+{
+    ARG_CHECK(out, "null output");
+    *out = nullptr;
+    ARG_CHECK(n == 0 || (d_positions && d_colors), "null positions or colours");
+    int rc = ensure_device();
+    if (rc) return rc;
+    sloth_mesh* m = new (std::nothrow) sloth_mesh();
+    if (!m) return fail(SLOTH_ENOMEM, "host allocation failed");
+    m->n = n;
+    m->d_xyz = d_positions;
+    m->d_rgb = d_colors;
+    if (n) {
+        std::vector<float> h;
+        try {
+            h.resize((size_t)n * 9);
+        } catch (...) {
+            delete m;
+            return fail(SLOTH_ENOMEM, "host allocation failed");
+        }
+        hipError_t e = hipMemcpy(h.data(), d_positions, (size_t)n * 36, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            delete m;
+            return fail(SLOTH_EHIP, "mesh bbox readback: %s", hipGetErrorString(e));
+        }
+        host_bbox(h.data(), n, m->bbmin, m->bbmax);
+    }
+    *out = m;
+    return SLOTH_OK;
+}
+
+int sloth_mesh_bbox(const sloth_mesh* mesh, float* min3, float* max3)
+{
+    ARG_CHECK(mesh, "null mesh");
+    if (min3) memcpy(min3, mesh->bbmin, 12);
+    if (max3) memcpy(max3, mesh->bbmax, 12);
+    return SLOTH_OK;
+}
+
+uint32_t sloth_mesh_triangles(const sloth_mesh* mesh) { return mesh ? mesh->n : 0; }
+
+int sloth_mesh_destroy(sloth_mesh* mesh)
+{
+    delete mesh;
+    return SLOTH_OK;
+}
+
+// ---- contexts -------------------------------------------------------------
+
+int sloth_context_create(uint32_t width, uint32_t height, int image, sloth_context** out)
+{
+    ARG_CHECK(out, "null output");
+    *out = nullptr;
+    ARG_CHECK(width >= 1 && width <= 65535 && height >= 1 && height <= 65535,
+              "width and height must be in 1..65535");
+    int rc = ensure_device();
+    if (rc) return rc;
+    sloth_context* c = new (std::nothrow) sloth_context();
+    if (!c) return fail(SLOTH_ENOMEM, "host allocation failed");
+    memcpy(c->U, IDENTITY, sizeof c->U);
+    c->job.fin.push_back(FrameIn{width, height, image ? 1 : 0});
+    c->stream = make_stream();
+    if (!c->stream) {
+        delete c;
+        return fail(SLOTH_EHIP, "hipStreamCreate failed");
+    }
+    *out = c;
+    return SLOTH_OK;
+}
+
+int sloth_context_destroy(sloth_context* ctx)
+{
+    if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return SLOTH_OK;
+}
+
+int sloth_context_update(sloth_context* ctx, const float* bbox_max, uint32_t n_meshes)
+{
+    ARG_CHECK(ctx, "null context");
+    ARG_CHECK(bbox_max || n_meshes == 0, "null bbox array");
+    screen_transform(ctx->job.fin[0].W, ctx->job.fin[0].H, bbox_max, n_meshes, ctx->U);
+    return SLOTH_OK;
+}
+
+int sloth_context_set_utransform(sloth_context* ctx, const float* m16)
+{
+    ARG_CHECK(ctx && m16, "null argument");
+    memcpy(ctx->U, m16, sizeof ctx->U);
+    return SLOTH_OK;
+}
+
+int sloth_context_get_utransform(const sloth_context* ctx, float* m16)
+{
+    ARG_CHECK(ctx && m16, "null argument");
+    memcpy(m16, ctx->U, sizeof ctx->U);
+    return SLOTH_OK;
+}
+
+int sloth_context_clear(sloth_context* ctx)
+{
+    ARG_CHECK(ctx, "null context");
+    ctx->job.din.clear();
+    ctx->job.ramps.clear();
+    ctx->rendered = false;
+    ctx->win_valid = false;
+    return SLOTH_OK;
+}
+
+int sloth_draw_mesh(sloth_context* ctx, const sloth_mesh* mesh, const float* transform16, const sloth_ramp* shader)
+{
+    ARG_CHECK(ctx && mesh && transform16, "null argument");
+    Ramp r;
+    int rc = to_ramp(shader, r);
+    if (rc) return rc;
+    DrawIn d;
+    d.frame = 0;
+    d.mesh = mesh;
+    mat4_mul(ctx->U, transform16, d.M);
+    d.ramp = ctx->job.add_ramp(r);
+    ctx->job.din.push_back(d);
+    ctx->rendered = false;
+    ctx->win_valid = false;
+    return SLOTH_OK;
+}
+
+static int context_render(sloth_context* ctx, hipStream_t s, bool want_win)
+{
+    int rc;
+    if (!ctx->rendered) {
+        if ((rc = ctx->job.layout())) return rc;
+        if ((rc = ctx->job.upload(ctx->stream))) return rc;
+        if (want_win && (rc = ctx->job.ensure_win())) return rc;
+        if ((rc = ctx->job.launch(0, 1, s, want_win ? ctx->job.d_win.as<int32_t>() : nullptr))) return rc;
+        ctx->rendered = true;
+        ctx->win_valid = want_win;
+    } else if (want_win && !ctx->win_valid) {
+        if ((rc = ctx->job.ensure_win())) return rc;
+        if ((rc = ctx->job.launch(0, 1, s, ctx->job.d_win.as<int32_t>()))) return rc;
+        ctx->win_valid = true;
+    }
+    return SLOTH_OK;
+}
+
+int sloth_context_render(sloth_context* ctx, void* stream)
+{
+    ARG_CHECK(ctx, "null context");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return context_render(ctx, stream ? (hipStream_t)stream : ctx->stream, false);
+}
+
+int sloth_context_read(sloth_context* ctx, uint32_t* cells, float* z, int32_t* winners)
+{
+    ARG_CHECK(ctx, "null context");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    if ((rc = context_render(ctx, s, winners != nullptr))) return rc;
+    const Job& j = ctx->job;
+    if (cells && j.n_cells)
+        HIP_TRY(hipMemcpyAsync(cells, j.d_cells.p, sizeof(uint32_t) * j.n_cells, hipMemcpyDeviceToHost, s));
+    if (z && j.n_z) HIP_TRY(hipMemcpyAsync(z, j.d_z.p, sizeof(float) * j.n_z, hipMemcpyDeviceToHost, s));
+    if (winners && j.n_z)
+        HIP_TRY(hipMemcpyAsync(winners, j.d_win.p, sizeof(int32_t) * j.n_z, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SLOTH_OK;
+}
+
+int sloth_context_device_buffers(sloth_context* ctx, uint32_t** d_cells, float** d_z, uint64_t* n_cells,
+                                 uint64_t* n_z)
+{
+    ARG_CHECK(ctx, "null context");
+    if (!ctx->job.laid_out) {
+        int rc = ctx->job.layout();
+        if (rc) return rc;
+    }
+    if (d_cells) *d_cells = ctx->job.d_cells.as<uint32_t>();
+    if (d_z) *d_z = ctx->job.d_z.as<float>();
+    if (n_cells) *n_cells = ctx->job.n_cells;
+    if (n_z) *n_z = ctx->job.n_z;
+    return SLOTH_OK;
+}
+
+// ---- batches --------------------------------------------------------------
+
+int sloth_batch_create(const sloth_frame_desc* frames, uint32_t n_frames, const sloth_draw_desc* draws,
+                       uint32_t n_draws, const sloth_ramp* shader, sloth_batch** out)
+{
+    ARG_CHECK(out, "null output");
+    *out = nullptr;
+    ARG_CHECK(frames && n_frames > 0, "a batch needs at least one frame");
+    ARG_CHECK(draws || n_draws == 0, "null draw array");
+    for (uint32_t f = 0; f < n_frames; ++f)
+        ARG_CHECK(frames[f].width >= 1 && frames[f].width <= 65535 && frames[f].height >= 1 &&
+                      frames[f].height <= 65535,
+                  "frame width and height must be in 1..65535");
+    for (uint32_t d = 0; d < n_draws; ++d)
+        ARG_CHECK(draws[d].frame < n_frames && draws[d].mesh, "draw with bad frame index or null mesh");
+    int rc = ensure_device();
+    if (rc) return rc;
+    Ramp r;
+    if ((rc = to_ramp(shader, r))) return rc;
+    sloth_batch* b = new (std::nothrow) sloth_batch();
+    if (!b) return fail(SLOTH_ENOMEM, "host allocation failed");
+    b->stream = make_stream();
+    if (!b->stream) {
+        delete b;
+        return fail(SLOTH_EHIP, "hipStreamCreate failed");
+    }
+    uint32_t ri = b->job.add_ramp(r);
+    b->U.resize((size_t)n_frames * 16);
+    for (uint32_t f = 0; f < n_frames; ++f) {
+        b->job.fin.push_back(FrameIn{frames[f].width, frames[f].height, frames[f].image ? 1 : 0});
+        memcpy(&b->U[(size_t)f * 16], frames[f].utransform, 64);
+    }
+    for (uint32_t d = 0; d < n_draws; ++d) {
+        DrawIn di;
+        di.frame = draws[d].frame;
+        di.mesh = draws[d].mesh;
+        mat4_mul(&b->U[(size_t)di.frame * 16], draws[d].transform, di.M);
+        di.ramp = ri;
+        b->job.din.push_back(di);
+    }
+    if ((rc = b->job.layout()) || (rc = b->job.upload(b->stream))) {
+        delete b;
+        return rc;
+    }
+    b->win_valid.assign(n_frames, false);
+    *out = b;
+    return SLOTH_OK;
+}
+
+int sloth_batch_set_transforms(sloth_batch* b, const float* transforms16)
+{
+    ARG_CHECK(b && transforms16, "null argument");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (size_t d = 0; d < b->job.din.size(); ++d) {
+        DrawIn& di = b->job.din[d];
+        mat4_mul(&b->U[(size_t)di.frame * 16], transforms16 + 16 * d, di.M);
+    }
+    // the draw table is grouped by frame; recompute it in the same order
+    int rc;
+    if ((rc = b->job.layout()) || (rc = b->job.upload(b->stream))) return rc;
+    std::fill(b->win_valid.begin(), b->win_valid.end(), false);
+    return SLOTH_OK;
+}
+
+int sloth_batch_run(sloth_batch* b, void* stream)
+{
+    ARG_CHECK(b, "null batch");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+    if (!b->exec) {
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = b->job.launch(0, (uint32_t)b->job.frames.size(), b->stream, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(SLOTH_EHIP, "graph capture: %s", hipGetErrorString(e));
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(b->exec, s));
+    b->ran = true;
+    std::fill(b->win_valid.begin(), b->win_valid.end(), false);
+    return SLOTH_OK;
+}
+
+int sloth_batch_run_per_frame(sloth_batch* b, void* stream)
+{
+    ARG_CHECK(b, "null batch");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+    for (uint32_t f = 0; f < (uint32_t)b->job.frames.size(); ++f) {
+        if ((rc = b->job.launch(f, f + 1, s, nullptr))) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    b->ran = true;
+    std::fill(b->win_valid.begin(), b->win_valid.end(), false);
+    return SLOTH_OK;
+}
+
+int sloth_batch_sync(sloth_batch* b)
+{
+    ARG_CHECK(b, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return SLOTH_OK;
+}
+
+int sloth_batch_read_frame(sloth_batch* b, uint32_t frame, uint32_t* cells, float* z, int32_t* winners)
+{
+    ARG_CHECK(b, "null batch");
+    ARG_CHECK(frame < b->job.frames.size(), "frame index out of range");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = b->stream;
+    HIP_TRY(hipDeviceSynchronize());
+    if (!b->ran) return fail(SLOTH_ESTATE, "batch has not been run");
+    const Frame& fr = b->job.frames[frame];
+    uint64_t WH = (uint64_t)fr.W * fr.H;
+    uint64_t nc = WH + (fr.image ? fr.H : 0);
+    if (winners && !b->win_valid[frame]) {
+        if ((rc = b->job.ensure_win())) return rc;
+        if ((rc = b->job.launch(frame, frame + 1, s, b->job.d_win.as<int32_t>()))) return rc;
+        b->win_valid[frame] = true;
+    }
+    if (cells)
+        HIP_TRY(hipMemcpyAsync(cells, b->job.d_cells.as<uint32_t>() + fr.cell_off, 4 * nc, hipMemcpyDeviceToHost, s));
+    if (z) HIP_TRY(hipMemcpyAsync(z, b->job.d_z.as<float>() + fr.z_off, 4 * WH, hipMemcpyDeviceToHost, s));
+    if (winners)
+        HIP_TRY(hipMemcpyAsync(winners, b->job.d_win.as<int32_t>() + fr.z_off, 4 * WH, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SLOTH_OK;
+}
+
+int sloth_batch_device_frame(sloth_batch* b, uint32_t frame, uint32_t** d_cells, float** d_z)
+{
+    ARG_CHECK(b, "null batch");
+    ARG_CHECK(frame < b->job.frames.size(), "frame index out of range");
+    const Frame& fr = b->job.frames[frame];
+    if (d_cells) *d_cells = b->job.d_cells.as<uint32_t>() + fr.cell_off;
+    if (d_z) *d_z = b->job.d_z.as<float>() + fr.z_off;
+    return SLOTH_OK;
+}
+
+int sloth_batch_profile(sloth_batch* b, float* out_ms, const char** names, uint32_t* n_stages)
+{
+    ARG_CHECK(b && n_stages, "null argument");
+    int rc = ensure_device();
+    if (rc) return rc;
+    const uint32_t NEV = 8;
+    std::vector<hipEvent_t> ev(NEV, nullptr);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    rc = b->job.launch(0, (uint32_t)b->job.frames.size(), b->stream, nullptr, &ev);
+    hipError_t se = hipStreamSynchronize(b->stream);
+    std::vector<const char*> nm;
+    if (b->job.path == PATH_TILED)
+        nm = TiledPlan::stage_names();
+    else
+        nm = {"k_clear_global", "k_raster_global", "k_resolve_global"};
+    uint32_t cap = *n_stages;
+    *n_stages = (uint32_t)nm.size();
+    if (!rc && se == hipSuccess) {
+        for (uint32_t i = 0; i < nm.size() && i < cap; ++i) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+            if (out_ms) out_ms[i] = ms;
+            if (names) names[i] = nm[i];
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    if (se != hipSuccess) return fail(SLOTH_EHIP, "profile run: %s", hipGetErrorString(se));
+    b->ran = true;
+    std::fill(b->win_valid.begin(), b->win_valid.end(), false);
+    return SLOTH_OK;
+}
+
+int sloth_batch_destroy(sloth_batch* b)
+{
+    if (b && b->stream) (void)hipStreamSynchronize(b->stream);
+    delete b;
+    return SLOTH_OK;
+}
+
+}  // extern "C"

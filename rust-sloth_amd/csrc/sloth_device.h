@@ -1,0 +1,283 @@
+// Device-side data layout and per-triangle arithmetic shared by every
+// raster kernel.  All floating-point expressions keep the reference's
+// binary32 operation order (rasterizer.rs, geometry.rs of ecumene/rust-sloth);
+// the library is compiled with -ffp-contract=off so none of them fuses.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sloth {
+
+constexpr uint64_t KEY_EMPTY = ~0ull;
+constexpr uint32_t CELL_BLANK = 0x20u;     // (' ', (0,0,0))   context.rs:36-37
+constexpr uint32_t CELL_NEWLINE = 0x0Au;   // ('\n', (0,0,0))  rasterizer.rs:90
+constexpr float F32_MAX = 3.40282347e+38f;
+constexpr int MAX_RAMP = 16;
+
+// A step shader: the first i with shade <= thr[i] gives glyph[i], else
+// fallback.  default_shader (rasterizer.rs:5-27) is the 9-step table.
+struct Ramp {
+    uint32_t n;
+    float thr[MAX_RAMP];
+    uint8_t glyph[MAX_RAMP];
+    uint8_t fallback;
+    uint8_t pad[3];
+};
+
+// One draw_mesh call (rasterizer.rs:39-46) inside one frame.
+struct Draw {
+    const float* xyz;     // [n][3][3] positions, w = 1
+    const uint8_t* rgb;   // [n][3]
+    uint32_t n;           // triangles
+    uint32_t tri_begin;   // frame-global index of the first triangle (tie order)
+    uint32_t frame;
+    uint32_t ramp;
+    uint64_t item_begin;  // index of the first triangle among all draws of the launch
+    float M[16];          // utransform * transform (rasterizer.rs:57), column-major
+};
+
+// One Context (context.rs:12-19) being rendered.
+struct Frame {
+    uint32_t W, H;
+    uint32_t image;
+    uint32_t draw_begin, draw_end;   // [begin, end) into the draw table
+    uint32_t n_tris;
+    uint64_t cell_off;    // into the cell buffer, W*H (+H) cells
+    uint64_t z_off;       // into the z / key / winner buffers, W*H slots
+    uint64_t nl_off;      // into the per-row newline table, H rows
+    uint64_t cell_begin;  // index of this frame's first cell among all frames
+};
+
+// Transformed triangle + the quantities draw_triangle derives from it.
+struct Tri {
+    float x1, y1, z1, x2, y2, z2, x3, y3, z3;
+    float A0, B0, A1, B1, A2, B2;   // edge deltas of orient(v2,v3,.), orient(v3,v1,.), orient(v1,v2,.)
+    float a;                        // 1 / orient(v1,v2,v3)      rasterizer.rs:67
+    float minx, maxx;
+    int x0, x1d, y0, y1d;           // pixel domain [x0,x1d) x [y0,y1d)  rasterizer.rs:58-66
+};
+
+// Matrix4 * Vector4 with w = 1 in nalgebra's gemv order (geometry.rs:44-46):
+// row i = (((M(i,0) x + M(i,1) y) + M(i,2) z) + M(i,3) * 1).
+__device__ __forceinline__ float xform_row(const float* M, int i, float x, float y, float z)
+{
+    float acc = M[0 + i] * x;
+    acc = M[4 + i] * y + acc;
+    acc = M[8 + i] * z + acc;
+    acc = M[12 + i] + acc;
+    return acc;
+}
+
+// `f as usize` (saturating, NaN -> 0), clamped to 2^30: every domain bound is
+// later compared against W-1 or H-1 < 2^16, so the clamp keeps empty ranges
+// empty.
+__device__ __forceinline__ int sat_index(float f)
+{
+    if (!(f > 0.0f)) return 0;
+    if (f >= 1073741824.0f) return 1 << 30;
+    return (int)f;
+}
+
+// Triangle::clone + mul + aabb + pixel domain + a (rasterizer.rs:56-67).
+__device__ __forceinline__ void setup_tri(const float* M, const float* p, uint32_t W, uint32_t H, Tri& t)
+{
+    t.x1 = xform_row(M, 0, p[0], p[1], p[2]);
+    t.y1 = xform_row(M, 1, p[0], p[1], p[2]);
+    t.z1 = xform_row(M, 2, p[0], p[1], p[2]);
+    t.x2 = xform_row(M, 0, p[3], p[4], p[5]);
+    t.y2 = xform_row(M, 1, p[3], p[4], p[5]);
+    t.z2 = xform_row(M, 2, p[3], p[4], p[5]);
+    t.x3 = xform_row(M, 0, p[6], p[7], p[8]);
+    t.y3 = xform_row(M, 1, p[6], p[7], p[8]);
+    t.z3 = xform_row(M, 2, p[6], p[7], p[8]);
+    t.minx = fminf(t.x1, fminf(t.x2, t.x3));
+    t.maxx = fmaxf(t.x1, fmaxf(t.x2, t.x3));
+    float miny = fminf(t.y1, fminf(t.y2, t.y3));
+    float maxy = fmaxf(t.y1, fmaxf(t.y2, t.y3));
+    t.x0 = sat_index(ceilf(fmaxf(t.minx, 1.0f)));
+    t.y0 = sat_index(ceilf(fmaxf(miny, 1.0f)));
+    t.x1d = sat_index(ceilf(fminf(t.maxx * 2.0f, (float)(W - 1))));
+    t.y1d = sat_index(ceilf(fminf(maxy, (float)(H - 1))));
+    t.A0 = t.x3 - t.x2;
+    t.B0 = t.y3 - t.y2;
+    t.A1 = t.x1 - t.x3;
+    t.B1 = t.y1 - t.y3;
+    t.A2 = t.x2 - t.x1;
+    t.B2 = t.y2 - t.y1;
+    float o = t.A2 * (t.y3 - t.y1) - t.B2 * (t.x3 - t.x1);
+    t.a = 1.0f / o;
+}
+
+// The three edge functions at integer pixel (x, y) (rasterizer.rs:71-74).
+__device__ __forceinline__ void edges(const Tri& t, float px, float py, float& w0, float& w1, float& w2)
+{
+    w0 = t.A0 * (py - t.y2) - t.B0 * (px - t.x2);
+    w1 = t.A1 * (py - t.y3) - t.B1 * (px - t.x3);
+    w2 = t.A2 * (py - t.y1) - t.B2 * (px - t.x1);
+}
+
+__device__ __forceinline__ bool inside(float w0, float w1, float w2)
+{
+    return w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f;
+}
+
+// Depth (rasterizer.rs:77-79).
+__device__ __forceinline__ float depth(const Tri& t, float w1, float w2)
+{
+    return t.z1 + t.a * (w1 * (t.z2 - t.z1) + w2 * (t.z3 - t.z1));
+}
+
+// z component of Triangle::normal() after Unit::new_normalize
+// (geometry.rs:49-56): products rounded before the subtraction, the squared
+// norm summed as ((x^2 + z^2) + (y^2 + 0^2)), then z / sqrt.
+__device__ __forceinline__ float unit_normal_z(const Tri& t)
+{
+    float e1x = t.x2 - t.x1, e1y = t.y2 - t.y1, e1z = t.z2 - t.z1;
+    float e2x = t.x3 - t.x1, e2y = t.y3 - t.y1, e2z = t.z3 - t.z1;
+    float nx = (e1y * e2z) - (e1z * e2y);
+    float ny = (e1z * e2x) - (e1x * e2z);
+    float nz = (e1x * e2y) - (e1y * e2x);
+    float a = nx * nx, b = ny * ny, c = nz * nz;
+    float d = 0.0f;
+    a += c;
+    b += d;
+    float n2 = 0.0f + (a + b);
+    return nz / sqrtf(n2);
+}
+
+// Flat shade value at a covered pixel (rasterizer.rs:76).
+__device__ __forceinline__ float shade_value(float nzu, float a, float w0, float w1, float w2)
+{
+    return nzu * a * (w0 + w1 + w2);
+}
+
+__device__ __forceinline__ uint8_t ramp_glyph(const Ramp& r, float s)
+{
+    for (uint32_t i = 0; i < r.n; ++i)
+        if (s <= r.thr[i]) return r.glyph[i];
+    return r.fallback;
+}
+
+// Order-preserving map of a depth that passed `z < f32::MAX` to 32 bits.
+// -0 and +0 compare equal in the reference's `<`, so both map to +0's code.
+__device__ __forceinline__ uint32_t depth_code(float z)
+{
+    uint32_t u = __float_as_uint(z);
+    if (z == 0.0f) u = 0u;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Packed z-test key: (depth code, frame triangle index, sub).  `sub` orders the
+// two logical pixels a z slot can receive from one triangle when 2x >= W wraps
+// id = y*W + 2x into the next row: the wrapped pixel (row y-1) is drawn first.
+__device__ __forceinline__ uint64_t make_key(float z, uint32_t tri, uint32_t sub)
+{
+    return ((uint64_t)depth_code(z) << 32) | (uint64_t)((tri << 1) | sub);
+}
+
+__device__ __forceinline__ uint32_t pack_cell(uint8_t g, const uint8_t* rgb)
+{
+    return (uint32_t)g | ((uint32_t)rgb[0] << 8) | ((uint32_t)rgb[1] << 16) | ((uint32_t)rgb[2] << 24);
+}
+
+// Logical pixel (x, y) that a key with `sub` denotes at z slot `id`.
+__device__ __forceinline__ void key_pixel(uint32_t id, uint32_t sub, uint32_t W, uint32_t& x, uint32_t& y)
+{
+    uint32_t row = id / W, col = id - row * W;
+    if (sub) {
+        y = row;
+        x = col >> 1;
+    } else {
+        y = row - 1;
+        x = (col + W) >> 1;
+    }
+}
+
+// Classification of a triangle's candidate pixels (see setup_class).
+enum : uint32_t { CLS_EMPTY = 0, CLS_BOX = 1, CLS_SPANS = 2, CLS_FULL = 3 };
+
+// Conservative proof that the reference's FP32 edge test cannot pass outside
+// the box [floor(minx), ceil(maxx)] (CLS_BOX) or anywhere at all (back face,
+// CLS_EMPTY).  Uses |w_fp - w_exact| <= 4.0001 u (|A| |py-ay| + |B| |px-ax|)
+// per edge over the whole pixel domain, and that the exact edge functions sum
+// to the exact orient(v1,v2,v3).  Triangles the bound cannot decide, and those
+// with coordinates outside +-2^22, fall to CLS_SPANS (exact per-row spans found
+// from the monotonicity of each FP32 edge function along a row) or CLS_FULL
+// (every pixel of the domain).  u = 2^-24; the factor 8 covers the bound's own
+// rounding.
+__device__ __forceinline__ uint32_t setup_class(const Tri& t)
+{
+    if (t.x0 >= t.x1d || t.y0 >= t.y1d) return CLS_EMPTY;
+    const float LIM = 4194304.0f;
+    bool finite = fabsf(t.x1) < LIM && fabsf(t.y1) < LIM && fabsf(t.x2) < LIM && fabsf(t.y2) < LIM &&
+                  fabsf(t.x3) < LIM && fabsf(t.y3) < LIM;
+    if (!finite) return CLS_FULL;
+    const float K = 8.0f * 5.9604645e-08f;
+    float X0 = (float)t.x0, X1 = (float)(t.x1d - 1), Y0 = (float)t.y0, Y1 = (float)(t.y1d - 1);
+    float dx0 = fmaxf(fabsf(X0 - t.x2), fabsf(X1 - t.x2)), dy0 = fmaxf(fabsf(Y0 - t.y2), fabsf(Y1 - t.y2));
+    float dx1 = fmaxf(fabsf(X0 - t.x3), fabsf(X1 - t.x3)), dy1 = fmaxf(fabsf(Y0 - t.y3), fabsf(Y1 - t.y3));
+    float dx2 = fmaxf(fabsf(X0 - t.x1), fabsf(X1 - t.x1)), dy2 = fmaxf(fabsf(Y0 - t.y1), fabsf(Y1 - t.y1));
+    float S = fabsf(t.A0) * dy0 + fabsf(t.B0) * dx0 + fabsf(t.A1) * dy1 + fabsf(t.B1) * dx1 +
+              fabsf(t.A2) * dy2 + fabsf(t.B2) * dx2;
+    float bound = K * S;
+    float ey = t.y3 - t.y1, ex = t.x3 - t.x1;
+    float o = t.A2 * ey - t.B2 * ex;
+    float eo = K * (fabsf(t.A2) * fabsf(ey) + fabsf(t.B2) * fabsf(ex));
+    if (o + eo < -bound) return CLS_EMPTY;          // back face, provably uncovered
+    float lo = o - eo;
+    if (lo > 0.0f && bound * (t.maxx - t.minx) < 0.5f * lo) return CLS_BOX;
+    return CLS_SPANS;
+}
+
+// Exact covered x-range on row y for CLS_SPANS triangles: each edge function
+// w_i(x) = fl(P_i - fl(B_i * fl(x - a_ix))) is monotone in x (rounding is
+// monotone), so {x : w_i(x) >= 0} is a prefix (B_i > 0), a suffix (B_i < 0) or
+// all/nothing (B_i == 0) of the row; binary search finds each boundary.
+// Returns false when the row has no covered pixel.
+__device__ __forceinline__ bool edge_bound(float A, float B, float ax, float ay, float py, int xa, int xb,
+                                           int& lo, int& hi)
+{
+    float P = A * (py - ay);
+    if (B == 0.0f) {
+        float w = P - B * ((float)xa - ax);
+        return w >= 0.0f;
+    }
+    if (B > 0.0f) {
+        // non-increasing: largest x in [lo,hi] with w >= 0
+        int l = lo, h = hi;
+        if (!(P - B * ((float)l - ax) >= 0.0f)) return false;
+        while (l < h) {
+            int m = l + ((h - l + 1) >> 1);
+            if (P - B * ((float)m - ax) >= 0.0f) l = m; else h = m - 1;
+        }
+        hi = l;
+        return true;
+    }
+    if (B < 0.0f) {
+        // non-decreasing: smallest x in [lo,hi] with w >= 0
+        int l = lo, h = hi;
+        if (!(P - B * ((float)h - ax) >= 0.0f)) return false;
+        while (l < h) {
+            int m = l + ((h - l) >> 1);
+            if (P - B * ((float)m - ax) >= 0.0f) h = m; else l = m + 1;
+        }
+        lo = l;
+        return true;
+    }
+    return false;   // NaN slope: never covered
+}
+
+__device__ __forceinline__ bool row_span(const Tri& t, int y, int& lo, int& hi)
+{
+    float py = (float)y;
+    lo = t.x0;
+    hi = t.x1d - 1;
+    if (!edge_bound(t.A0, t.B0, t.x2, t.y2, py, t.x0, t.x1d - 1, lo, hi)) return false;
+    if (lo > hi) return false;
+    if (!edge_bound(t.A1, t.B1, t.x3, t.y3, py, t.x0, t.x1d - 1, lo, hi)) return false;
+    if (lo > hi) return false;
+    if (!edge_bound(t.A2, t.B2, t.x1, t.y1, py, t.x0, t.x1d - 1, lo, hi)) return false;
+    return lo <= hi;
+}
+
+}  // namespace sloth

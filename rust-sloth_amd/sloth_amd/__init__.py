@@ -1,0 +1,425 @@
+"""Python mirror of sloth's raster API over libsloth_hip.so (ctypes).
+
+Names follow the reference (ecumene/rust-sloth):
+
+    SimpleMesh      geometry.rs:78-81   triangle soup + bounding_box
+    Context         context.rs:12-142   blank / clear / update / utransform,
+                                        frame_buffer / z_buffer
+    draw_mesh       rasterizer.rs:39-46
+    default_shader  rasterizer.rs:5-27  (a step table, see Shader)
+
+The rasterization runs on the GPU through the C-ABI in include/sloth_hip.h.
+There is no CPU fallback: importing works without a GPU, but creating a
+Context or SimpleMesh raises SlothError if the library or the device is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import meshgen  # noqa: F401  (synthetic workloads)
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libsloth_hip.so")
+
+SLOTH_OK, SLOTH_EINVAL, SLOTH_ENOMEM, SLOTH_EHIP, SLOTH_ENODEV, SLOTH_ESTATE = range(6)
+_ERRNAMES = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ENODEV", 5: "ESTATE"}
+
+
+class SlothError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sloth error {_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class sloth_ramp(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("thresholds", C.c_float * 16), ("glyphs", C.c_uint8 * 16),
+                ("fallback", C.c_uint8), ("reserved", C.c_uint8 * 3)]
+
+
+class sloth_frame_desc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("image", C.c_int32),
+                ("utransform", C.c_float * 16)]
+
+
+class sloth_draw_desc(C.Structure):
+    _fields_ = [("frame", C.c_uint32), ("mesh", C.c_void_p), ("transform", C.c_float * 16)]
+
+
+# exported symbol -> (restype, argtypes); the CPU tests check this list against
+# include/sloth_hip.h
+_fp = C.POINTER(C.c_float)
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+_vpp = C.POINTER(C.c_void_p)
+SIGNATURES = {
+    "sloth_init": (C.c_int, [C.c_int]),
+    "sloth_last_error": (C.c_char_p, []),
+    "sloth_default_ramp": (C.c_int, [C.POINTER(sloth_ramp)]),
+    "sloth_mesh_create": (C.c_int, [_fp, _u8p, C.c_uint32, _vpp]),
+    "sloth_mesh_create_device": (C.c_int, [_vp, _vp, C.c_uint32, _vpp]),
+    "sloth_mesh_bbox": (C.c_int, [_vp, _fp, _fp]),
+    "sloth_mesh_triangles": (C.c_uint32, [_vp]),
+    "sloth_mesh_destroy": (C.c_int, [_vp]),
+    "sloth_context_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, _vpp]),
+    "sloth_context_destroy": (C.c_int, [_vp]),
+    "sloth_context_update": (C.c_int, [_vp, _fp, C.c_uint32]),
+    "sloth_context_set_utransform": (C.c_int, [_vp, _fp]),
+    "sloth_context_get_utransform": (C.c_int, [_vp, _fp]),
+    "sloth_context_clear": (C.c_int, [_vp]),
+    "sloth_draw_mesh": (C.c_int, [_vp, _vp, _fp, C.POINTER(sloth_ramp)]),
+    "sloth_context_render": (C.c_int, [_vp, _vp]),
+    "sloth_context_read": (C.c_int, [_vp, _u32p, _fp, _i32p]),
+    "sloth_context_device_buffers": (C.c_int, [_vp, _vpp, _vpp, C.POINTER(C.c_uint64),
+                                               C.POINTER(C.c_uint64)]),
+    "sloth_batch_create": (C.c_int, [C.POINTER(sloth_frame_desc), C.c_uint32,
+                                     C.POINTER(sloth_draw_desc), C.c_uint32,
+                                     C.POINTER(sloth_ramp), _vpp]),
+    "sloth_batch_set_transforms": (C.c_int, [_vp, _fp]),
+    "sloth_batch_run": (C.c_int, [_vp, _vp]),
+    "sloth_batch_run_per_frame": (C.c_int, [_vp, _vp]),
+    "sloth_batch_sync": (C.c_int, [_vp]),
+    "sloth_batch_read_frame": (C.c_int, [_vp, C.c_uint32, _u32p, _fp, _i32p]),
+    "sloth_batch_device_frame": (C.c_int, [_vp, C.c_uint32, _vpp, _vpp]),
+    "sloth_batch_profile": (C.c_int, [_vp, _fp, C.POINTER(C.c_char_p), _u32p]),
+    "sloth_batch_destroy": (C.c_int, [_vp]),
+    "sloth_screen_transform": (C.c_int, [C.c_uint32, C.c_uint32, _fp, C.c_uint32, _fp]),
+    "sloth_mat4_mul": (C.c_int, [_fp, _fp, _fp]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libsloth_hip.so and declares every exported signature.  Raises
+    SlothError if the library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SlothError(SLOTH_ENODEV, f"{path} is missing: build it first (__graft_entry__.build())")
+    L = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != SLOTH_OK:
+        raise SlothError(rc, load_library().sloth_last_error().decode(errors="replace"))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
+
+
+_initialised = False
+
+
+def init(device: int = 0):
+    """Selects and checks the GPU (gfx950)."""
+    global _initialised
+    _check(load_library().sloth_init(device))
+    _initialised = True
+
+
+def _ensure_init():
+    if not _initialised:
+        init(0)
+
+
+# ---------------------------------------------------------------------------
+# shaders
+# ---------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class Shader:
+    """A step shader: first i with shade <= thresholds[i] -> glyphs[i], else
+    fallback (NaN included).  The reference's `Fn(f32) -> char` closures
+    (rasterizer.rs:39-42) of that form."""
+
+    thresholds: tuple
+    glyphs: str
+    fallback: str = " "
+
+    def to_c(self) -> sloth_ramp:
+        if len(self.thresholds) != len(self.glyphs) or len(self.glyphs) > 16:
+            raise ValueError("a shader needs equally many thresholds and glyphs, at most 16")
+        r = sloth_ramp()
+        r.n = len(self.thresholds)
+        for i, (t, g) in enumerate(zip(self.thresholds, self.glyphs)):
+            r.thresholds[i] = float(np.float32(t))
+            r.glyphs[i] = ord(g)
+        r.fallback = ord(self.fallback)
+        return r
+
+    def __call__(self, shade: float) -> str:
+        s = np.float32(shade)
+        for t, g in zip(self.thresholds, self.glyphs):
+            if s <= np.float32(t):
+                return g
+        return self.fallback
+
+
+default_shader = Shader((0.20, 0.30, 0.40, 0.50, 0.60, 0.70, 0.80, 0.90, 1.0), ".:-=+*#%@", " ")
+
+
+# ---------------------------------------------------------------------------
+# SimpleMesh
+# ---------------------------------------------------------------------------
+
+@dataclass
+class AABB:
+    """geometry.rs:5-15 (xyz; w is always 1)."""
+
+    min: np.ndarray
+    max: np.ndarray
+
+
+class SimpleMesh:
+    """geometry.rs:78-81: a device-resident triangle soup.
+
+    positions: [n,3,3] float32 (v1, v2, v3 as x,y,z; w = 1), colors: [n,3]
+    uint8.  `bounding_box` follows the OBJ loader (geometry.rs:85-88,
+    129-135)."""
+
+    def __init__(self, positions, colors):
+        _ensure_init()
+        L = load_library()
+        self.positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3, 3)
+        self.colors = np.ascontiguousarray(colors, np.uint8).reshape(-1, 3)
+        if self.positions.shape[0] != self.colors.shape[0]:
+            raise ValueError("positions and colors disagree on the triangle count")
+        h = C.c_void_p()
+        _check(L.sloth_mesh_create(_fptr(self.positions), self.colors.ctypes.data_as(_u8p),
+                                   self.positions.shape[0], C.byref(h)))
+        self._h = h
+        mn = np.zeros(3, np.float32)
+        mx = np.zeros(3, np.float32)
+        _check(L.sloth_mesh_bbox(h, _fptr(mn), _fptr(mx)))
+        self.bounding_box = AABB(mn, mx)
+
+    @classmethod
+    def from_mesh(cls, m) -> "SimpleMesh":
+        return cls(m.xyz, m.rgb)
+
+    @property
+    def n(self) -> int:
+        return int(self.positions.shape[0])
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.sloth_mesh_destroy(h)
+            self._h = None
+
+
+# ---------------------------------------------------------------------------
+# Context
+# ---------------------------------------------------------------------------
+
+class Context:
+    """context.rs:12-142 on the GPU.
+
+    The frame buffer is exposed as packed uint32 cells (byte0 glyph, bytes
+    1..3 R,G,B); `frame_buffer` decodes it to the reference's
+    (char, (r,g,b)) list."""
+
+    def __init__(self, width: int, height: int, image: bool = True):
+        _ensure_init()
+        L = load_library()
+        h = C.c_void_p()
+        _check(L.sloth_context_create(width, height, int(bool(image)), C.byref(h)))
+        self._h = h
+        self.width, self.height, self.image = int(width), int(height), bool(image)
+        self._meshes = []          # keep drawn meshes alive until clear
+
+    @classmethod
+    def blank(cls, image: bool, width: int, height: int) -> "Context":
+        """Context::blank (context.rs:22-34) with image mode's fixed size."""
+        return cls(width, height, image)
+
+    @property
+    def utransform(self) -> np.ndarray:
+        m = np.empty(16, np.float32)
+        _check(load_library().sloth_context_get_utransform(self._h, _fptr(m)))
+        return m
+
+    @utransform.setter
+    def utransform(self, m):
+        m = np.ascontiguousarray(m, np.float32).reshape(16)
+        _check(load_library().sloth_context_set_utransform(self._h, _fptr(m)))
+
+    def update(self, meshes) -> None:
+        """Context::update (context.rs:93-141)."""
+        bm = np.ascontiguousarray(np.stack([m.bounding_box.max for m in meshes]) if meshes
+                                  else np.zeros((0, 3), np.float32), np.float32)
+        _check(load_library().sloth_context_update(self._h, _fptr(bm), bm.shape[0]))
+
+    def clear(self) -> None:
+        """Context::clear (context.rs:35-45)."""
+        _check(load_library().sloth_context_clear(self._h))
+        self._meshes = []
+
+    def draw(self, mesh: SimpleMesh, transform, shader: Shader = default_shader) -> None:
+        t = np.ascontiguousarray(transform, np.float32).reshape(16)
+        r = shader.to_c()
+        _check(load_library().sloth_draw_mesh(self._h, mesh.handle, _fptr(t), C.byref(r)))
+        self._meshes.append(mesh)
+
+    def render(self, stream: int | None = None) -> None:
+        _check(load_library().sloth_context_render(self._h, C.c_void_p(stream or 0)))
+
+    def read(self, want_winners: bool = False):
+        """(cells uint32, z float32, winners int32 or None)."""
+        nc = self.width * self.height + (self.height if self.image else 0)
+        cells = np.empty(nc, np.uint32)
+        z = np.empty(self.width * self.height, np.float32)
+        win = np.empty(self.width * self.height, np.int32) if want_winners else None
+        _check(load_library().sloth_context_read(
+            self._h, cells.ctypes.data_as(_u32p), _fptr(z),
+            win.ctypes.data_as(_i32p) if want_winners else None))
+        return cells, z, win
+
+    @property
+    def z_buffer(self) -> np.ndarray:
+        return self.read()[1]
+
+    @property
+    def frame_buffer(self) -> list:
+        cells = self.read()[0]
+        return [(chr(c & 0xFF), ((c >> 8) & 0xFF, (c >> 16) & 0xFF, (c >> 24) & 0xFF)) for c in cells]
+
+    def text(self) -> str:
+        """The no-colour flush (context.rs:58-61) without the trailing newline."""
+        return (self.read()[0] & 0xFF).astype(np.uint8).tobytes().decode("latin-1")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.sloth_context_destroy(h)
+            self._h = None
+
+
+def draw_mesh(context: Context, mesh: SimpleMesh, transform, shader: Shader = default_shader) -> None:
+    """rasterizer.rs:39-46."""
+    context.draw(mesh, transform, shader)
+
+
+def screen_transform(width: int, height: int, bbox_maxes) -> np.ndarray:
+    bm = np.ascontiguousarray(np.asarray(bbox_maxes, np.float32).reshape(-1, 3))
+    out = np.empty(16, np.float32)
+    _check(load_library().sloth_screen_transform(width, height, _fptr(bm), bm.shape[0], _fptr(out)))
+    return out
+
+
+def mat4_mul(a, b) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float32).reshape(16)
+    b = np.ascontiguousarray(b, np.float32).reshape(16)
+    out = np.empty(16, np.float32)
+    _check(load_library().sloth_mat4_mul(_fptr(a), _fptr(b), _fptr(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# batches
+# ---------------------------------------------------------------------------
+
+@dataclass
+class FrameSpec:
+    width: int
+    height: int
+    utransform: np.ndarray
+    image: bool = True
+
+
+@dataclass
+class DrawSpec:
+    frame: int
+    mesh: SimpleMesh
+    transform: np.ndarray
+
+
+class Batch:
+    """Many frames rendered by one launch sequence (captured in a HIP graph)."""
+
+    def __init__(self, frames: list, draws: list, shader: Shader = default_shader):
+        _ensure_init()
+        L = load_library()
+        fa = (sloth_frame_desc * len(frames))()
+        for i, f in enumerate(frames):
+            fa[i].width, fa[i].height, fa[i].image = f.width, f.height, int(bool(f.image))
+            u = np.ascontiguousarray(f.utransform, np.float32).reshape(16)
+            for k in range(16):
+                fa[i].utransform[k] = float(u[k])
+        da = (sloth_draw_desc * max(1, len(draws)))()
+        for i, d in enumerate(draws):
+            da[i].frame = d.frame
+            da[i].mesh = d.mesh.handle.value
+            t = np.ascontiguousarray(d.transform, np.float32).reshape(16)
+            for k in range(16):
+                da[i].transform[k] = float(t[k])
+        r = shader.to_c()
+        h = C.c_void_p()
+        _check(L.sloth_batch_create(fa, len(frames), da, len(draws), C.byref(r), C.byref(h)))
+        self._h = h
+        self.frames = frames
+        self.draws = draws
+        self._keep = [d.mesh for d in draws]
+
+    def set_transforms(self, transforms) -> None:
+        t = np.ascontiguousarray(transforms, np.float32).reshape(-1, 16)
+        _check(load_library().sloth_batch_set_transforms(self._h, _fptr(t)))
+
+    def run(self, stream: int | None = None) -> None:
+        _check(load_library().sloth_batch_run(self._h, C.c_void_p(stream or 0)))
+
+    def run_per_frame(self, stream: int | None = None) -> None:
+        _check(load_library().sloth_batch_run_per_frame(self._h, C.c_void_p(stream or 0)))
+
+    def sync(self) -> None:
+        _check(load_library().sloth_batch_sync(self._h))
+
+    def read_frame(self, f: int, want_winners: bool = False):
+        fr = self.frames[f]
+        nc = fr.width * fr.height + (fr.height if fr.image else 0)
+        cells = np.empty(nc, np.uint32)
+        z = np.empty(fr.width * fr.height, np.float32)
+        win = np.empty(fr.width * fr.height, np.int32) if want_winners else None
+        _check(load_library().sloth_batch_read_frame(
+            self._h, f, cells.ctypes.data_as(_u32p), _fptr(z),
+            win.ctypes.data_as(_i32p) if want_winners else None))
+        return cells, z, win
+
+    def device_frame(self, f: int):
+        c = C.c_void_p()
+        z = C.c_void_p()
+        _check(load_library().sloth_batch_device_frame(self._h, f, C.byref(c), C.byref(z)))
+        return c.value, z.value
+
+    def profile(self) -> dict:
+        cap = 16
+        ms = (C.c_float * cap)()
+        names = (C.c_char_p * cap)()
+        n = C.c_uint32(cap)
+        _check(load_library().sloth_batch_profile(self._h, ms, names, C.byref(n)))
+        return {names[i].decode(): float(ms[i]) for i in range(min(n.value, cap))}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.sloth_batch_destroy(h)
+            self._h = None

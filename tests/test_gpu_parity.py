@@ -1,0 +1,244 @@
+"""GPU raster vs the CPU oracle, bit for bit: frame-buffer cells (glyph +
+RGB), z buffer bits and the winning triangle of every z slot.
+
+Every case is small enough for the oracle to finish in seconds; the
+full-size workloads are covered in test_gpu_fullsize.py."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from _helpers import assert_frame_equal, gpu_render, oracle_render, random_soup, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(sloth, name, W, H, U, draws, image=True):
+    say(f"case {name}: {W}x{H} image={image} draws={len(draws)} "
+        f"tris={sum(d[0].shape[0] for d in draws)}")
+    cells, z, win = gpu_render(sloth, W, H, U, draws, image=image)
+    ref = oracle_render(W, H, U, draws, image=image)
+    assert_frame_equal(cells, z, win, ref, W)
+    return ref
+
+
+def test_sphere_160x48(sloth):
+    from sloth_amd import meshgen
+
+    m, W, H, U, R = meshgen.workload_sphere_160x48()
+    ref = _check(sloth, "uv-sphere 960", W, H, U, [(m.xyz, m.rgb, R, None)])
+    assert ref.stats[2] > 0
+
+
+def test_sphere_terminal_mode(sloth):
+    from sloth_amd import meshgen
+
+    m, W, H, U, R = meshgen.workload_sphere_160x48()
+    _check(sloth, "uv-sphere terminal", W, H, U, [(m.xyz, m.rgb, R, None)], image=False)
+
+
+@pytest.mark.parametrize("W,H", [(160, 48), (37, 23), (101, 64), (64, 64), (2, 2), (1, 5), (5, 1), (3, 3)])
+def test_random_soups(sloth, W, H):
+    rng = np.random.default_rng(W * 1000 + H)
+    from sloth_amd import meshgen
+
+    U = meshgen.screen_transform(W, H, [np.array([1.0, 1.0, 1.0], np.float32)])
+    draws = []
+    for k in range(3):
+        xyz, rgb = random_soup(rng, 400, W, H, U, spread=1.4, size=0.08 * (k + 1))
+        R = meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3))
+        draws.append((xyz, rgb, R, None))
+    _check(sloth, f"random soups {W}x{H}", W, H, U, draws)
+
+
+def test_custom_shaders_per_draw(sloth):
+    from sloth_amd import meshgen, Shader
+
+    rng = np.random.default_rng(7)
+    W, H = 120, 40
+    U = meshgen.screen_transform(W, H, [np.array([1.0, 1.0, 1.0], np.float32)])
+    sh1 = Shader((0.5, 0.9), "ab", "z")
+    sh2 = Shader((-0.1, 0.25, 0.75, 1.0, 2.0), "01234", "#")
+    draws = []
+    for sh in (sh1, sh2, None):
+        xyz, rgb = random_soup(rng, 300, W, H, U, size=0.1)
+        draws.append((xyz, rgb, meshgen.rotation_euler(0.2, 1.1, 0.3), sh))
+    _check(sloth, "per-draw shaders", W, H, U, draws)
+
+
+def test_wraparound_aliasing(sloth):
+    """Logical x >= W/2 maps into the next row (id = y*W + 2x,
+    rasterizer.rs:80); triangles there and the newline cells they overwrite."""
+    from sloth_amd import meshgen
+
+    rng = np.random.default_rng(11)
+    for W, H in ((80, 30), (81, 30)):
+        U = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1], np.float32)
+        # triangles directly in screen space, right half of the logical range
+        n = 600
+        cx = rng.uniform(W / 2 - 4, W - 1, n)
+        cy = rng.uniform(0, H, n)
+        xyz = np.stack([np.stack([cx + rng.normal(0, 2, n), cy + rng.normal(0, 2, n),
+                                  rng.uniform(-5, 5, n)], 1) for _ in range(3)], 1).astype(np.float32)
+        rgb = rng.integers(0, 256, (n, 3), dtype=np.uint8)
+        _check(sloth, f"wrap {W}x{H}", W, H, U, [(xyz, rgb, U, None)])
+
+
+def test_degenerate_and_extreme(sloth):
+    """Points, segments, collinear slivers, horizontal lines on integer rows,
+    NaN and infinite vertices, huge coordinates, full-screen triangles."""
+    W, H = 64, 32
+    I = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1], np.float32)
+    tris = [
+        [[10, 10, 1], [10, 10, 1], [10, 10, 1]],            # point
+        [[5, 7, 2], [20, 7, 2], [12, 7, 2]],                # horizontal line on a row
+        [[5, 8, 2], [5, 8, 2], [30, 8, 3]],                 # segment with repeated vertex
+        [[3, 3, 0], [13, 9, 0], [23, 15, 0]],               # collinear
+        [[0, 0, -1], [0, 40, -1], [80, 0, -1]],             # covers the screen, front
+        [[0, 0, -2], [80, 0, -2], [0, 40, -2]],             # same, back-facing
+        [[1e30, 1, 0], [2, 1e30, 0], [3, 3, 0]],            # huge
+        [[np.nan, 4, 0], [10, 20, 0], [20, 4, 0]],          # NaN vertex
+        [[np.inf, 4, 0], [10, 20, 0], [20, 4, 0]],          # inf vertex
+        [[-np.inf, 4, 0], [10, 20, 0], [20, 4, 0]],
+        [[10, 4, np.nan], [10, 20, 0], [20, 4, 0]],         # NaN depth
+        [[30, 10, 0], [31, 10.5, 0], [30.5, 11, 0]],        # sub-pixel
+        [[40, 5, 5], [60, 5, 5], [40, 25, 5]],              # right-angle, integer
+        [[40, 5, 5], [40, 25, 5], [60, 5, 5]],              # same, other winding
+        [[40, 5, 5], [60, 5, 5], [40, 25, 5]],              # exact duplicate (tie)
+        [[-10, -10, 0], [-5, -10, 0], [-10, -5, 0]],        # off screen
+        [[1, 1, 3.4e38], [30, 1, 3.4e38], [1, 30, 3.4e38]], # depth at f32::MAX
+        [[2, 2, -0.0], [20, 2, 0.0], [2, 20, -0.0]],        # signed zero depth
+        [[2, 2, 0.0], [20, 2, 0.0], [2, 20, 0.0]],
+        [[31.999999, 3, 1], [32.000001, 3, 1], [32, 29, 1]],  # needle at the wrap column
+    ]
+    xyz = np.array(tris, np.float32)
+    rgb = (np.arange(len(tris) * 3).reshape(-1, 3) * 37 % 256).astype(np.uint8)
+    _check(sloth, "degenerate/extreme", W, H, I, [(xyz, rgb, I, None)])
+    _check(sloth, "degenerate/extreme reversed", W, H, I, [(xyz[::-1].copy(), rgb[::-1].copy(), I, None)])
+
+
+def test_slivers_near_silhouette(sloth):
+    """Nearly edge-on triangles (orient close to 0), where the FP32 edge test
+    can disagree with exact arithmetic."""
+    rng = np.random.default_rng(5)
+    W, H = 200, 60
+    I = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1], np.float32)
+    n = 2000
+    p = rng.uniform([2, 2], [98, 58], (n, 2))
+    d = rng.normal(0, 6, (n, 2))
+    t = rng.uniform(0.2, 0.8, n)
+    eps = rng.normal(0, 1e-4, (n, 2))
+    v1 = p
+    v2 = p + d
+    v3 = p + t[:, None] * d + eps
+    z = rng.uniform(-10, 10, (n, 3, 1))
+    xyz = np.concatenate([np.stack([v1, v2, v3], 1), z], 2).astype(np.float32)
+    rgb = rng.integers(0, 256, (n, 3), dtype=np.uint8)
+    _check(sloth, "slivers", W, H, I, [(xyz, rgb, I, None)])
+
+
+def test_coplanar_ties(sloth):
+    """Identical triangles drawn twice: the first one drawn keeps every z slot
+    (strict `<`, rasterizer.rs:81)."""
+    from sloth_amd import meshgen
+
+    m = meshgen.uv_sphere(16, 8, seed=4)
+    W, H = 90, 40
+    U = meshgen.screen_transform(W, H, [m.bbox_max()])
+    R = meshgen.rotation_euler(0.0, 3.5, 0.0)
+    rgb2 = (255 - m.rgb).astype(np.uint8)
+    ref = _check(sloth, "ties", W, H, U, [(m.xyz, m.rgb, R, None), (m.xyz, rgb2, R, None)])
+    assert (ref.win[ref.win >= 0] < m.n).all()
+
+
+def test_skull_frames(sloth):
+    """The reference's skull.obj (4 meshes, 3185 triangles) at 100x100 for a
+    sample of the `-j 100` turntable frames, every mesh a separate draw."""
+    import os
+
+    from sloth_amd import meshgen
+
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "skull_datajs.npz"))
+    xyz, rgb, counts = d["xyz"], d["rgb"], d["mesh_counts"]
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    W, H = int(d["width"]), int(d["height"])
+    U = meshgen.screen_transform(W, H, [oracle.mesh_bbox(xyz[offs[i]:offs[i + 1]])[1]
+                                        for i in range(len(counts))])
+    pitches = meshgen.turntable_pitches(100)
+    for k in (0, 17, 50, 99):
+        R = oracle.rotation_euler(0.0, pitches[k], 0.0)
+        draws = [(xyz[offs[i]:offs[i + 1]], rgb[offs[i]:offs[i + 1]], R, None) for i in range(len(counts))]
+        _check(sloth, f"skull frame {k}", W, H, U, draws)
+
+
+def test_batch_matches_oracle(sloth):
+    """A batch of frames of different sizes, draws interleaved across frames,
+    rendered by the graph launch and by the per-frame launches."""
+    from sloth_amd import meshgen
+
+    rng = np.random.default_rng(21)
+    sizes = [(80, 24), (64, 64), (37, 19), (160, 48), (2, 2)]
+    meshes = [meshgen.uv_sphere(12, 6, seed=1), meshgen.displaced_icosphere(2, seed=2),
+              meshgen.subdivided_box(3, seed=3)]
+    smeshes = [sloth.SimpleMesh.from_mesh(m) for m in meshes]
+    frames, draws, plan = [], [], []
+    for f, (W, H) in enumerate(sizes):
+        U = meshgen.screen_transform(W, H, [m.bbox_max() for m in meshes])
+        frames.append(sloth.FrameSpec(W, H, U, image=(f % 2 == 0)))
+    for k in range(9):
+        f = int(rng.integers(0, len(sizes)))
+        mi = int(rng.integers(0, len(meshes)))
+        R = meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3))
+        draws.append(sloth.DrawSpec(f, smeshes[mi], R))
+        plan.append((f, mi, R))
+    b = sloth.Batch(frames, draws)
+    for mode in ("graph", "per-frame", "graph-again"):
+        say(f"case batch {mode}")
+        if mode == "per-frame":
+            b.run_per_frame()
+        else:
+            b.run()
+        b.sync()
+        for f, fs in enumerate(frames):
+            fd = [(meshes[mi].xyz, meshes[mi].rgb, R, None) for (ff, mi, R) in plan if ff == f]
+            ref = oracle_render(fs.width, fs.height, fs.utransform, fd, image=fs.image)
+            cells, z, win = b.read_frame(f, want_winners=True)
+            assert_frame_equal(cells, z, win, ref, fs.width)
+
+
+def test_context_redraw_after_read(sloth):
+    """draw, read, draw more, read: the second read covers every draw since
+    the clear, in draw order."""
+    from sloth_amd import meshgen
+
+    m1 = meshgen.uv_sphere(16, 8, seed=1)
+    m2 = meshgen.subdivided_box(2, seed=2)
+    W, H = 70, 30
+    U = meshgen.screen_transform(W, H, [m1.bbox_max(), m2.bbox_max()])
+    R1 = meshgen.rotation_euler(0.1, 0.7, 0.0)
+    R2 = meshgen.rotation_euler(0.0, 2.0, 0.4)
+    ctx = sloth.Context(W, H, image=True)
+    ctx.utransform = U
+    ctx.clear()
+    g1, g2 = sloth.SimpleMesh.from_mesh(m1), sloth.SimpleMesh.from_mesh(m2)
+    sloth.draw_mesh(ctx, g1, R1)
+    c1, z1, w1 = ctx.read(want_winners=True)
+    ref = oracle_render(W, H, U, [(m1.xyz, m1.rgb, R1, None)])
+    assert_frame_equal(c1, z1, w1, ref, W)
+    sloth.draw_mesh(ctx, g2, R2)
+    c2, z2, w2 = ctx.read(want_winners=True)
+    ref = oracle_render(W, H, U, [(m1.xyz, m1.rgb, R1, None), (m2.xyz, m2.rgb, R2, None)])
+    assert_frame_equal(c2, z2, w2, ref, W)
+    ctx.clear()
+    c3, z3, _ = ctx.read()
+    assert (c3 == oracle.CELL_BLANK).all() and (z3 == oracle.F32_MAX).all()
+
+
+def test_empty_mesh_and_frame(sloth):
+    W, H = 10, 6
+    I = np.eye(4, dtype=np.float32).reshape(16)
+    _check(sloth, "empty mesh", W, H, I, [(np.zeros((0, 3, 3), np.float32), np.zeros((0, 3), np.uint8), I, None)])
+    _check(sloth, "no draws", W, H, I, [])
