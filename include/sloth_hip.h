@@ -82,7 +82,7 @@ int sloth_mesh_create(const float *positions, const uint8_t *colors, uint32_t n,
 
 /* Wraps positions/colours that already live in device memory (no copy; the
  * caller keeps them alive and unchanged while the mesh is used).  The
- * bounding box is computed on the device. */
+ * bounding box is computed from a copy read back to the host. */
 int sloth_mesh_create_device(const float *d_positions, const uint8_t *d_colors, uint32_t n,
                              sloth_mesh **out);
 
@@ -183,6 +183,13 @@ int sloth_batch_device_frame(sloth_batch *b, uint32_t frame, uint32_t **d_cells,
  * stream: out_ms[i] = milliseconds of stage i, names[i] its kernel name;
  * *n_stages in: capacity, out: count. */
 int sloth_batch_profile(sloth_batch *b, float *out_ms, const char **names, uint32_t *n_stages);
+
+/* Runs `warmup` untimed renders, then `steps` timed renders bracketed by HIP
+ * events on the batch's stream, and waits.  mode 0: graph replay
+ * (sloth_batch_run); mode 1: one launch sequence per frame with a device
+ * synchronisation after each (sloth_batch_run_per_frame).  *ms = elapsed
+ * milliseconds of the timed renders. */
+int sloth_batch_time(sloth_batch *b, uint32_t steps, uint32_t warmup, int mode, float *ms);
 
 int sloth_batch_destroy(sloth_batch *b);
 

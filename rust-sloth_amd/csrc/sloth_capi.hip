@@ -884,6 +884,33 @@ int sloth_batch_profile(sloth_batch* b, float* out_ms, const char** names, uint3
     return SLOTH_OK;
 }
 
+int sloth_batch_time(sloth_batch* b, uint32_t steps, uint32_t warmup, int mode, float* ms)
+{
+    ARG_CHECK(b && ms, "null argument");
+    ARG_CHECK(mode == 0 || mode == 1, "mode must be 0 (graph) or 1 (per frame)");
+    int rc = ensure_device();
+    if (rc) return rc;
+    for (uint32_t i = 0; i < warmup; ++i)
+        if ((rc = mode == 0 ? sloth_batch_run(b, nullptr) : sloth_batch_run_per_frame(b, nullptr))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, b->stream));
+    for (uint32_t i = 0; i < steps && rc == SLOTH_OK; ++i)
+        rc = mode == 0 ? sloth_batch_run(b, nullptr) : sloth_batch_run_per_frame(b, nullptr);
+    hipError_t e = hipEventRecord(e1, b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SLOTH_EHIP, "batch timing: %s", hipGetErrorString(e));
+    *ms = t;
+    return SLOTH_OK;
+}
+
 int sloth_batch_destroy(sloth_batch* b)
 {
     if (b && b->stream) (void)hipStreamSynchronize(b->stream);

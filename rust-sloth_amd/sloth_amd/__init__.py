@@ -88,6 +88,7 @@ SIGNATURES = {
     "sloth_batch_read_frame": (C.c_int, [_vp, C.c_uint32, _u32p, _fp, _i32p]),
     "sloth_batch_device_frame": (C.c_int, [_vp, C.c_uint32, _vpp, _vpp]),
     "sloth_batch_profile": (C.c_int, [_vp, _fp, C.POINTER(C.c_char_p), _u32p]),
+    "sloth_batch_time": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int, _fp]),
     "sloth_batch_destroy": (C.c_int, [_vp]),
     "sloth_screen_transform": (C.c_int, [C.c_uint32, C.c_uint32, _fp, C.c_uint32, _fp]),
     "sloth_mat4_mul": (C.c_int, [_fp, _fp, _fp]),
@@ -409,6 +410,13 @@ class Batch:
         z = C.c_void_p()
         _check(load_library().sloth_batch_device_frame(self._h, f, C.byref(c), C.byref(z)))
         return c.value, z.value
+
+    def time(self, steps: int, warmup: int, per_frame: bool = False) -> float:
+        """Milliseconds for `steps` renders after `warmup` untimed ones (HIP
+        events on the batch's stream)."""
+        ms = C.c_float()
+        _check(load_library().sloth_batch_time(self._h, steps, warmup, int(per_frame), C.byref(ms)))
+        return float(ms.value)
 
     def profile(self) -> dict:
         cap = 16
