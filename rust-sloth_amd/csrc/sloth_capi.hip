@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "sloth_hip.h"
+#include "sloth_host.h"
 #include "sloth_kernels.h"
 #include "sloth_tiled.h"
 
@@ -27,7 +28,7 @@ using namespace sloth;
 static thread_local std::string g_err;
 static int g_device = 0;
 
-static int fail(int code, const char* fmt, ...)
+int sloth::fail(int code, const char* fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -37,17 +38,6 @@ static int fail(int code, const char* fmt, ...)
     g_err = buf;
     return code;
 }
-
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess) return fail(SLOTH_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-#define ARG_CHECK(cond, msg) \
-    do {                     \
-        if (!(cond)) return fail(SLOTH_EINVAL, "%s", msg); \
-    } while (0)
 
 // ---------------------------------------------------------------------------
 // host arithmetic in the reference's order (compiled with -ffp-contract=off)
@@ -126,37 +116,6 @@ static bool same_ramp(const Ramp& a, const Ramp& b)
         if (memcmp(&a.thr[i], &b.thr[i], 4) != 0 || a.glyph[i] != b.glyph[i]) return false;
     return true;
 }
-
-// ---------------------------------------------------------------------------
-// device buffers
-// ---------------------------------------------------------------------------
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    // Grow-only allocation; contents are not preserved.
-    int reserve(size_t n)
-    {
-        if (n <= bytes && p) return SLOTH_OK;
-        release();
-        if (n == 0) return SLOTH_OK;
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) {
-            p = nullptr;
-            (void)hipGetLastError();
-            return fail(SLOTH_ENOMEM, "hipMalloc(%zu bytes): %s", n, hipGetErrorString(e));
-        }
-        bytes = n;
-        return SLOTH_OK;
-    }
-    template <class T> T* as() const { return (T*)p; }
-};
 
 // ---------------------------------------------------------------------------
 // meshes
@@ -270,6 +229,7 @@ struct Job {
             fr.n_tris = tri;
             uint64_t WH = (uint64_t)fr.W * fr.H;
             cells += WH + (fr.image ? fr.H : 0);
+            cells = (cells + 1) & ~1ull;   // even offsets: slots store cell pairs as one 8-byte word
             z += WH;
             rows += fr.H;
             if (fr.W & 1u) all_even = false;
@@ -278,8 +238,7 @@ struct Job {
         n_z = z;
         n_rows = rows;
         n_items = items;
-        path = PATH_GLOBAL;
-        (void)all_even;
+        path = (all_even && !force_global()) ? PATH_TILED : PATH_GLOBAL;
         int rc;
         if ((rc = d_frames.reserve(sizeof(Frame) * std::max<size_t>(1, frames.size())))) return rc;
         if ((rc = d_draws.reserve(sizeof(Draw) * std::max<size_t>(1, draws.size())))) return rc;
@@ -290,7 +249,7 @@ struct Job {
             if ((rc = d_keys.reserve(sizeof(uint64_t) * n_z))) return rc;
             if ((rc = d_nl.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_rows)))) return rc;
         } else {
-            if ((rc = tiled.plan(frames, draws, n_rows))) return rc;
+            if ((rc = tiled.plan(frames, n_rows))) return rc;
         }
         laid_out = true;
         return SLOTH_OK;
@@ -362,6 +321,35 @@ struct Job {
         return SLOTH_OK;
     }
 
+    // Work that must precede a launch and cannot be captured in a graph: the
+    // counting run that sizes the tile lists.
+    int prepare(hipStream_t s)
+    {
+        if (path == PATH_TILED && !tiled.sized)
+            return tiled.size(frames, draws, d_frames.as<Frame>(), d_draws.as<Draw>(), n_items, s);
+        return SLOTH_OK;
+    }
+
+    // After a render of frames [f_lo, f_hi) completed on `s`: if a tile list
+    // overflowed, re-size the lists and render those frames again.
+    // *replanned tells the caller that device pointers changed.
+    int finish(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, bool* replanned = nullptr)
+    {
+        if (replanned) *replanned = false;
+        if (path != PATH_TILED) return SLOTH_OK;
+        for (int attempt = 0; attempt < 4; ++attempt) {
+            bool ovf = false;
+            int rc = tiled.overflowed((uint32_t)frames.size(), s, ovf);
+            if (rc) return rc;
+            if (!ovf) return SLOTH_OK;
+            tiled.sized = false;
+            if (replanned) *replanned = true;
+            if ((rc = prepare(s))) return rc;
+            if ((rc = launch(f_lo, f_hi, s, win))) return rc;
+        }
+        return fail(SLOTH_EHIP, "tile lists kept overflowing");
+    }
+
     int launch(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, std::vector<hipEvent_t>* ev = nullptr)
     {
         if (f_lo >= f_hi) return SLOTH_OK;
@@ -396,6 +384,7 @@ struct sloth_context {
     hipStream_t stream = nullptr;
     bool rendered = false;      // device buffers match the draw list
     bool win_valid = false;     // d_win matches the draw list
+    bool checked = false;       // the last render was checked for tile-list overflow
     ~sloth_context()
     {
         if (stream) (void)hipStreamDestroy(stream);
@@ -411,8 +400,10 @@ struct sloth_batch {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t exec_generation = 0;   // tile plan generation the graph was captured with
     bool ran = false;
     std::vector<bool> win_valid;
+    bool checked = false;           // the last render was checked for tile-list overflow
     ~sloth_batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -652,13 +643,16 @@ static int context_render(sloth_context* ctx, hipStream_t s, bool want_win)
     if (!ctx->rendered) {
         if ((rc = ctx->job.layout())) return rc;
         if ((rc = ctx->job.upload(ctx->stream))) return rc;
+        if ((rc = ctx->job.prepare(ctx->stream))) return rc;
         if (want_win && (rc = ctx->job.ensure_win())) return rc;
         if ((rc = ctx->job.launch(0, 1, s, want_win ? ctx->job.d_win.as<int32_t>() : nullptr))) return rc;
         ctx->rendered = true;
+        ctx->checked = false;
         ctx->win_valid = want_win;
     } else if (want_win && !ctx->win_valid) {
         if ((rc = ctx->job.ensure_win())) return rc;
         if ((rc = ctx->job.launch(0, 1, s, ctx->job.d_win.as<int32_t>()))) return rc;
+        ctx->checked = false;
         ctx->win_valid = true;
     }
     return SLOTH_OK;
@@ -679,6 +673,10 @@ int sloth_context_read(sloth_context* ctx, uint32_t* cells, float* z, int32_t* w
     if (rc) return rc;
     hipStream_t s = ctx->stream;
     if ((rc = context_render(ctx, s, winners != nullptr))) return rc;
+    if (!ctx->checked) {
+        if ((rc = ctx->job.finish(0, 1, s, ctx->win_valid ? ctx->job.d_win.as<int32_t>() : nullptr))) return rc;
+        ctx->checked = true;
+    }
     const Job& j = ctx->job;
     if (cells && j.n_cells)
         HIP_TRY(hipMemcpyAsync(cells, j.d_cells.p, sizeof(uint32_t) * j.n_cells, hipMemcpyDeviceToHost, s));
@@ -768,12 +766,32 @@ int sloth_batch_set_transforms(sloth_batch* b, const float* transforms16)
     return SLOTH_OK;
 }
 
+static void drop_graph(sloth_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Checks the finished renders for tile-list overflow and re-renders if needed.
+static int batch_finish(sloth_batch* b)
+{
+    bool replanned = false;
+    int rc = b->job.finish(0, (uint32_t)b->job.frames.size(), b->stream, nullptr, &replanned);
+    if (rc) return rc;
+    if (replanned) drop_graph(b);
+    return SLOTH_OK;
+}
+
 int sloth_batch_run(sloth_batch* b, void* stream)
 {
     ARG_CHECK(b, "null batch");
     int rc = ensure_device();
     if (rc) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+    if ((rc = b->job.prepare(b->stream))) return rc;
+    if (b->exec && b->exec_generation != b->job.tiled.generation) drop_graph(b);
     if (!b->exec) {
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         rc = b->job.launch(0, (uint32_t)b->job.frames.size(), b->stream, nullptr);
@@ -786,9 +804,11 @@ int sloth_batch_run(sloth_batch* b, void* stream)
         if (e != hipSuccess) return fail(SLOTH_EHIP, "graph capture: %s", hipGetErrorString(e));
         b->graph = g;
         HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->exec_generation = b->job.tiled.generation;
     }
     HIP_TRY(hipGraphLaunch(b->exec, s));
     b->ran = true;
+    b->checked = false;
     std::fill(b->win_valid.begin(), b->win_valid.end(), false);
     return SLOTH_OK;
 }
@@ -799,11 +819,13 @@ int sloth_batch_run_per_frame(sloth_batch* b, void* stream)
     int rc = ensure_device();
     if (rc) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+    if ((rc = b->job.prepare(b->stream))) return rc;
     for (uint32_t f = 0; f < (uint32_t)b->job.frames.size(); ++f) {
         if ((rc = b->job.launch(f, f + 1, s, nullptr))) return rc;
         HIP_TRY(hipStreamSynchronize(s));
     }
     b->ran = true;
+    b->checked = false;
     std::fill(b->win_valid.begin(), b->win_valid.end(), false);
     return SLOTH_OK;
 }
@@ -812,6 +834,11 @@ int sloth_batch_sync(sloth_batch* b)
 {
     ARG_CHECK(b, "null batch");
     HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->ran && !b->checked) {
+        int rc = batch_finish(b);
+        if (rc) return rc;
+        b->checked = true;
+    }
     return SLOTH_OK;
 }
 
@@ -822,8 +849,8 @@ int sloth_batch_read_frame(sloth_batch* b, uint32_t frame, uint32_t* cells, floa
     int rc = ensure_device();
     if (rc) return rc;
     hipStream_t s = b->stream;
-    HIP_TRY(hipDeviceSynchronize());
     if (!b->ran) return fail(SLOTH_ESTATE, "batch has not been run");
+    if ((rc = sloth_batch_sync(b))) return rc;
     const Frame& fr = b->job.frames[frame];
     uint64_t WH = (uint64_t)fr.W * fr.H;
     uint64_t nc = WH + (fr.image ? fr.H : 0);
@@ -859,7 +886,8 @@ int sloth_batch_profile(sloth_batch* b, float* out_ms, const char** names, uint3
     const uint32_t NEV = 8;
     std::vector<hipEvent_t> ev(NEV, nullptr);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-    rc = b->job.launch(0, (uint32_t)b->job.frames.size(), b->stream, nullptr, &ev);
+    rc = b->job.prepare(b->stream);
+    if (!rc) rc = b->job.launch(0, (uint32_t)b->job.frames.size(), b->stream, nullptr, &ev);
     hipError_t se = hipStreamSynchronize(b->stream);
     std::vector<const char*> nm;
     if (b->job.path == PATH_TILED)
@@ -880,7 +908,15 @@ int sloth_batch_profile(sloth_batch* b, float* out_ms, const char** names, uint3
     if (rc) return rc;
     if (se != hipSuccess) return fail(SLOTH_EHIP, "profile run: %s", hipGetErrorString(se));
     b->ran = true;
+    b->checked = false;
     std::fill(b->win_valid.begin(), b->win_valid.end(), false);
+    bool replanned = false;
+    if ((rc = b->job.finish(0, (uint32_t)b->job.frames.size(), b->stream, nullptr, &replanned))) return rc;
+    b->checked = true;
+    if (replanned) {
+        drop_graph(b);
+        return fail(SLOTH_ESTATE, "tile lists were re-sized during profiling; profile again");
+    }
     return SLOTH_OK;
 }
 
@@ -890,25 +926,34 @@ int sloth_batch_time(sloth_batch* b, uint32_t steps, uint32_t warmup, int mode, 
     ARG_CHECK(mode == 0 || mode == 1, "mode must be 0 (graph) or 1 (per frame)");
     int rc = ensure_device();
     if (rc) return rc;
-    for (uint32_t i = 0; i < warmup; ++i)
-        if ((rc = mode == 0 ? sloth_batch_run(b, nullptr) : sloth_batch_run_per_frame(b, nullptr))) return rc;
-    HIP_TRY(hipStreamSynchronize(b->stream));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, b->stream));
-    for (uint32_t i = 0; i < steps && rc == SLOTH_OK; ++i)
-        rc = mode == 0 ? sloth_batch_run(b, nullptr) : sloth_batch_run_per_frame(b, nullptr);
-    hipError_t e = hipEventRecord(e1, b->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float t = 0.0f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(SLOTH_EHIP, "batch timing: %s", hipGetErrorString(e));
-    *ms = t;
-    return SLOTH_OK;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        for (uint32_t i = 0; i < warmup; ++i)
+            if ((rc = mode == 0 ? sloth_batch_run(b, nullptr) : sloth_batch_run_per_frame(b, nullptr))) return rc;
+        if ((rc = sloth_batch_sync(b))) return rc;
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, b->stream));
+        for (uint32_t i = 0; i < steps && rc == SLOTH_OK; ++i)
+            rc = mode == 0 ? sloth_batch_run(b, nullptr) : sloth_batch_run_per_frame(b, nullptr);
+        hipError_t e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(SLOTH_EHIP, "batch timing: %s", hipGetErrorString(e));
+        bool replanned = false;
+        if ((rc = b->job.finish(0, (uint32_t)b->job.frames.size(), b->stream, nullptr, &replanned))) return rc;
+        b->checked = true;
+        if (!replanned) {
+            *ms = t;
+            return SLOTH_OK;
+        }
+        drop_graph(b);   // tile lists were re-sized: the timed renders were redone; time again
+    }
+    return fail(SLOTH_ESTATE, "tile lists kept re-sizing while timing");
 }
 
 int sloth_batch_destroy(sloth_batch* b)

@@ -46,6 +46,8 @@ struct Frame {
     uint64_t z_off;       // into the z / key / winner buffers, W*H slots
     uint64_t nl_off;      // into the per-row newline table, H rows
     uint64_t cell_begin;  // index of this frame's first cell among all frames
+    uint32_t tiles_x, tiles_y;   // tiled path: tile grid over the W x H cells
+    uint64_t tile_begin;         // tiled path: index of the frame's first tile
 };
 
 // Transformed triangle + the quantities draw_triangle derives from it.
@@ -278,6 +280,42 @@ __device__ __forceinline__ bool row_span(const Tri& t, int y, int& lo, int& hi)
     if (lo > hi) return false;
     if (!edge_bound(t.A2, t.B2, t.x1, t.y1, py, t.x0, t.x1d - 1, lo, hi)) return false;
     return lo <= hi;
+}
+
+// Candidate box [bx0,bx1] x [by0,by1] (logical pixels, inclusive) that holds
+// every pixel of the triangle's domain the FP32 edge test can pass; false when
+// there is none.  CLS_BOX uses the proof in setup_class, CLS_SPANS the union of
+// the exact row spans, CLS_FULL the whole domain.
+__device__ __forceinline__ bool tri_box(const Tri& t, uint32_t cls, int& bx0, int& bx1, int& by0, int& by1)
+{
+    if (cls == CLS_EMPTY) return false;
+    by0 = t.y0;
+    by1 = t.y1d - 1;
+    if (cls == CLS_BOX) {
+        bx0 = max(t.x0, (int)floorf(t.minx));
+        bx1 = min(t.x1d - 1, (int)ceilf(t.maxx));
+        return bx0 <= bx1;
+    }
+    if (cls == CLS_FULL) {
+        bx0 = t.x0;
+        bx1 = t.x1d - 1;
+        return true;
+    }
+    int lo_all = 1 << 30, hi_all = -1, first = -1, last = -1;
+    for (int y = t.y0; y < t.y1d; ++y) {
+        int lo, hi;
+        if (!row_span(t, y, lo, hi)) continue;
+        lo_all = min(lo_all, lo);
+        hi_all = max(hi_all, hi);
+        if (first < 0) first = y;
+        last = y;
+    }
+    if (first < 0) return false;
+    bx0 = lo_all;
+    bx1 = hi_all;
+    by0 = first;
+    by1 = last;
+    return true;
 }
 
 }  // namespace sloth

@@ -1,22 +1,71 @@
-// Tiled raster path (even widths): declared here, implemented in sloth_tiled.hip.
+// Tiled raster path for frames of even width (internal).
+//
+// The W x H cell grid of every frame is cut into TW x TH cell tiles (TW even,
+// tiles start on even columns).  With W even, z slot ids = y*W + 2x are even,
+// so a tile's slots and the two cells each slot writes (id, id+1) never cross
+// a tile edge, and a tile can resolve its cells alone.
+//
+//   k_tile_clear   tile cursors <- 0, newline rows <- -1, overflow <- 0,
+//                  image-mode tail cells <- blank
+//   k_bin          one thread per triangle: setup, classification, candidate
+//                  box, newline rows (atomicMax), and the triangle index
+//                  appended to every tile its box reaches, with one atomicAdd
+//                  per distinct tile per wavefront (ballot + prefix count)
+//   k_tile_raster  one workgroup per tile: 64-bit LDS atomicMin of the packed
+//                  keys, a second walk in which each slot's winning fragment
+//                  stages its cell and depth in LDS, then coalesced stores
+//
+// Tile lists live in one pool at fixed per-tile offsets, sized from an exact
+// counting run of k_bin plus headroom.  If a later run would exceed a tile's
+// capacity (the transforms changed), k_bin raises the overflow flag and the
+// host re-plans and renders again.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <vector>
 
 #include "sloth_device.h"
+#include "sloth_host.h"
 
 namespace sloth {
 
 struct TiledPlan {
-    int plan(const std::vector<Frame>&, const std::vector<Draw>&, uint64_t) { return 5; }
-    int upload(hipStream_t) { return 5; }
-    int launch(const std::vector<Frame>&, const std::vector<Draw>&, const Frame*, const Draw*, const Ramp*, uint32_t,
-               uint32_t, uint32_t*, float*, int32_t*, hipStream_t, std::vector<hipEvent_t>*)
-    {
-        return 5;
-    }
-    static std::vector<const char*> stage_names() { return {}; }
+    uint32_t TW = 64, TH = 16;       // tile size in cells (TW even)
+    uint64_t n_tiles = 0;
+    uint64_t n_rows = 0;
+    bool sized = false;              // pool offsets match the current layout
+    uint64_t generation = 0;         // bumped whenever size() runs (device pointers may change)
+    std::vector<uint64_t> off;       // host copy of the per-tile pool offsets (n_tiles + 1)
+    DevBuf d_cursor, d_off, d_pool, d_nl, d_overflow;
+
+    // Sets the tile grid of every frame and allocates per-tile and per-row
+    // state.  Pool offsets become invalid until size() runs.
+    int plan(std::vector<Frame>& frames, uint64_t rows);
+
+    // Counting run over all frames (synchronous): per-tile list lengths ->
+    // pool offsets with headroom.
+    int size(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
+             const Draw* d_draws, uint64_t n_items, hipStream_t s);
+
+    int upload(hipStream_t s) { (void)s; return SLOTH_OK; }
+
+    // clear + bin + raster over frames [f_lo, f_hi).  ev (optional) gets 4
+    // events recorded around the 3 stages.
+    int launch(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
+               const Draw* d_draws, const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells, float* z,
+               int32_t* win, hipStream_t s, std::vector<hipEvent_t>* ev);
+
+    // Reads the overflow flags of frames [0, nf) (synchronises the stream).
+    int overflowed(uint32_t nf, hipStream_t s, bool& ovf);
+
+    static std::vector<const char*> stage_names() { return {"k_tile_clear", "k_bin", "k_tile_raster"}; }
+
+  private:
+    int launch_clear(const std::vector<Frame>& frames, const Frame* d_frames, uint32_t f_lo, uint32_t f_hi,
+                     uint32_t* cells, hipStream_t s);
+    int launch_bin(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
+                   const Draw* d_draws, uint32_t f_lo, uint32_t f_hi, uint64_t n_items, bool count_only,
+                   hipStream_t s);
 };
 
 }  // namespace sloth

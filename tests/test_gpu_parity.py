@@ -16,8 +16,22 @@ from _helpers import assert_frame_equal, gpu_render, oracle_render, random_soup,
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=["tiled", "global"])
+def raster_path(request, monkeypatch):
+    """Every case runs on the tiled path (even widths) and on the general
+    global-key path (forced by SLOTH_FORCE_GLOBAL=1)."""
+    if request.param == "global":
+        monkeypatch.setenv("SLOTH_FORCE_GLOBAL", "1")
+    else:
+        monkeypatch.delenv("SLOTH_FORCE_GLOBAL", raising=False)
+    return request.param
+
+
 def _check(sloth, name, W, H, U, draws, image=True):
-    say(f"case {name}: {W}x{H} image={image} draws={len(draws)} "
+    import os
+
+    path = "global" if os.environ.get("SLOTH_FORCE_GLOBAL") == "1" or W % 2 else "tiled"
+    say(f"case {name} [{path}]: {W}x{H} image={image} draws={len(draws)} "
         f"tris={sum(d[0].shape[0] for d in draws)}")
     cells, z, win = gpu_render(sloth, W, H, U, draws, image=image)
     ref = oracle_render(W, H, U, draws, image=image)
