@@ -49,7 +49,11 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--profile-runs", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--graph-only", action="store_true",
+                    help="only the batched graph renders (for profilers): no plain, stage or CPU legs")
     args = ap.parse_args()
+    if args.graph_only:
+        args.plain_steps = args.profile_runs = args.cpu_frames = 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:

@@ -8,7 +8,7 @@
 namespace sloth {
 
 constexpr int BIN_BLOCK = 256;
-constexpr int RASTER_BLOCK = 256;
+constexpr int RASTER_BLOCK = 64;
 
 // ---------------------------------------------------------------------------
 // k_tile_clear
@@ -105,12 +105,14 @@ __device__ __forceinline__ bool tile_at(const TileRects& r, int k, int& tx, int&
     return !(nd > 0 && tx >= r.dx0 && tx <= r.dx1 && ty >= r.dy0 && ty <= r.dy1);
 }
 
+// COUNT = true is the sizing run: it only counts each tile's entries.
+template <bool COUNT>
 __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draws, uint32_t d_lo, uint32_t d_hi,
                                                    const Frame* __restrict__ frames, uint64_t item_base,
                                                    uint64_t n_items, uint32_t TW, uint32_t TH,
                                                    uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
                                                    uint32_t* __restrict__ pool, int32_t* __restrict__ nl,
-                                                   uint32_t* __restrict__ overflow, int count_only)
+                                                   uint32_t* __restrict__ overflow)
 {
     const uint64_t t_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = t_id < n_items;
@@ -126,9 +128,16 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     r.dy0 = r.wy0 = 1;
     r.dy1 = r.wy1 = 0;
 
+    // Blocks rarely straddle two draws: if this one does not, the draw index is
+    // uniform and its fields (the 4x4 matrix included) come in scalar loads.
+    const uint64_t blk0 = item_base + (uint64_t)blockIdx.x * blockDim.x;
+    const uint64_t blk1 = item_base + min((uint64_t)blockIdx.x * blockDim.x + blockDim.x, n_items) - 1;
+    const uint32_t d_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)find_draw_t(draws, d_lo, d_hi, blk0));
+    const uint32_t d_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)find_draw_t(draws, d_lo, d_hi, blk1));
+    int y0 = 0, y1d = 0;
     if (valid) {
         uint64_t item = item_base + t_id;
-        uint32_t d = find_draw_t(draws, d_lo, d_hi, item);
+        uint32_t d = d_first == d_last ? d_first : find_draw_t(draws, d_lo, d_hi, item);
         const Draw& dr = draws[d];
         uint32_t local = (uint32_t)(item - dr.item_begin);
         frame = dr.frame;
@@ -136,10 +145,9 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         Tri t;
         setup_tri(dr.M, dr.xyz + (size_t)local * 9, fr.W, fr.H, t);
         tri = dr.tri_begin + local;
-        if (fr.image && t.y0 < t.y1d) {
-            // rasterizer.rs:89-91: a newline cell after every row of the domain
-            int32_t* fnl = nl + fr.nl_off;
-            for (int y = t.y0; y < t.y1d; ++y) atomicMax(&fnl[y], (int32_t)tri);
+        if (fr.image) {
+            y0 = t.y0;
+            y1d = t.y1d;
         }
         uint32_t cls = setup_class(t);
         int bx0, bx1, by0, by1;
@@ -149,6 +157,41 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         }
         tile_begin = fr.tile_begin;
         tiles_x = fr.tiles_x;
+    }
+
+    // rasterizer.rs:89-91: a newline cell after every row of the domain; the
+    // last triangle (in draw order) whose rows include r decides row r.  When
+    // the wave is in one frame and its rows span < 64, one ballot per row finds
+    // the highest lane (= latest triangle) covering it and one atomicMax
+    // publishes it; otherwise every lane publishes its own rows.
+    {
+        const bool has_rows = valid && y0 < y1d;
+        const uint64_t rmask = __ballot(has_rows);
+        if (rmask) {
+            const int l0 = __ffsll((unsigned long long)rmask) - 1;
+            const uint32_t f0 = (uint32_t)__builtin_amdgcn_readlane((int)frame, l0);
+            const bool same = __ballot(has_rows && frame != f0) == 0;
+            int lo_r = has_rows ? y0 : (1 << 30), hi_r = has_rows ? y1d - 1 : -1;
+            for (int sh = 32; sh >= 1; sh >>= 1) {
+                lo_r = min(lo_r, __shfl_xor(lo_r, sh));
+                hi_r = max(hi_r, __shfl_xor(hi_r, sh));
+            }
+            lo_r = __builtin_amdgcn_readfirstlane(lo_r);
+            hi_r = __builtin_amdgcn_readfirstlane(hi_r);
+            int32_t* fnl = nl + frames[f0].nl_off;
+            if (same && hi_r - lo_r < 64) {
+                for (int row = lo_r; row <= hi_r; ++row) {
+                    uint64_t m = __ballot(has_rows && y0 <= row && row < y1d);
+                    if (m) {
+                        int top = 63 - __clzll((long long)m);
+                        if (lane == top) atomicMax(&fnl[row], (int32_t)tri);
+                    }
+                }
+            } else if (has_rows) {
+                int32_t* mine = nl + frames[frame].nl_off;
+                for (int y = y0; y < y1d; ++y) atomicMax(&mine[y], (int32_t)tri);
+            }
+        }
     }
 
     // Append `tri` to each of its tiles; lanes that share a tile share one
@@ -176,7 +219,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
         if (match) {
             uint32_t pos = base + (uint32_t)__popcll(mmask & __lanemask_lt());
-            if (!count_only) {
+            if (!COUNT) {
                 uint64_t o = off[lk], cap = off[lk + 1] - o;
                 if (pos < cap) pool[o + pos] = tri;
                 else overflow[frame] = 1u;
@@ -260,15 +303,13 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
     const uint32_t* list = pool + o;
     __syncthreads();
 
+    const bool one_draw = fr.draw_end - fr.draw_begin == 1;
+    auto draw_for = [&](uint32_t T) -> const Draw& {
+        return one_draw ? draws[fr.draw_begin] : draw_of_t(draws, fr, T);
+    };
+
     // pass 1: depth test of every candidate fragment
-    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-        const uint32_t T = list[e];
-        const Draw& dr = draw_of_t(draws, fr, T);
-        const uint32_t li = T - dr.tri_begin;
-        Tri t;
-        setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
-        int bx0, bx1, by0, by1;
-        if (!tri_box(t, setup_class(t), bx0, bx1, by0, by1)) continue;
+    auto test = [&](const Tri& t, uint32_t T, int bx0, int bx1, int by0, int by1) {
         walk_tile(g, bx0, bx1, by0, by1, [&](int x, int y, int slot, uint32_t sub) {
             float w0, w1, w2;
             edges(t, (float)x, (float)y, w0, w1, w2);
@@ -277,18 +318,10 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
             if (!(z < F32_MAX)) return;
             atomicMin((unsigned long long*)&keys[slot], (unsigned long long)make_key(z, T, sub));
         });
-    }
-    __syncthreads();
-
+    };
     // pass 2: the fragment whose key won stages its cell and depth
-    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-        const uint32_t T = list[e];
-        const Draw& dr = draw_of_t(draws, fr, T);
+    auto stage = [&](const Tri& t, uint32_t T, const Draw& dr, int bx0, int bx1, int by0, int by1) {
         const uint32_t li = T - dr.tri_begin;
-        Tri t;
-        setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
-        int bx0, bx1, by0, by1;
-        if (!tri_box(t, setup_class(t), bx0, bx1, by0, by1)) continue;
         bool have_nz = false;
         float nzu = 0.0f;
         uint32_t color = 0;
@@ -309,6 +342,37 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
             st_cell[slot] = (uint32_t)ramp_glyph(ramps[dr.ramp], s) | color;
             st_z[slot] = z;
         });
+    };
+
+    // The first entry of each thread stays in registers for both passes.
+    Tri t0;
+    uint32_t T0 = 0;
+    int c0x0 = 0, c0x1 = -1, c0y0 = 0, c0y1 = -1;
+    bool ok0 = false;
+    if (threadIdx.x < n) {
+        T0 = list[threadIdx.x];
+        const Draw& dr = draw_for(T0);
+        setup_tri(dr.M, dr.xyz + (size_t)(T0 - dr.tri_begin) * 9, W, H, t0);
+        ok0 = tri_box(t0, setup_class(t0), c0x0, c0x1, c0y0, c0y1);
+    }
+    if (ok0) test(t0, T0, c0x0, c0x1, c0y0, c0y1);
+    for (uint32_t e = threadIdx.x + blockDim.x; e < n; e += blockDim.x) {
+        const uint32_t T = list[e];
+        const Draw& dr = draw_for(T);
+        Tri t;
+        setup_tri(dr.M, dr.xyz + (size_t)(T - dr.tri_begin) * 9, W, H, t);
+        int bx0, bx1, by0, by1;
+        if (tri_box(t, setup_class(t), bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
+    }
+    __syncthreads();
+    if (ok0) stage(t0, T0, draw_for(T0), c0x0, c0x1, c0y0, c0y1);
+    for (uint32_t e = threadIdx.x + blockDim.x; e < n; e += blockDim.x) {
+        const uint32_t T = list[e];
+        const Draw& dr = draw_for(T);
+        Tri t;
+        setup_tri(dr.M, dr.xyz + (size_t)(T - dr.tri_begin) * 9, W, H, t);
+        int bx0, bx1, by0, by1;
+        if (tri_box(t, setup_class(t), bx0, bx1, by0, by1)) stage(t, T, dr, bx0, bx1, by0, by1);
     }
     __syncthreads();
 
@@ -397,9 +461,14 @@ int TiledPlan::launch_bin(const std::vector<Frame>& frames, const std::vector<Dr
     uint64_t it_lo = d_lo < draws.size() ? draws[d_lo].item_begin : n_items;
     uint64_t it_hi = d_hi < draws.size() ? draws[d_hi].item_begin : n_items;
     if (it_hi <= it_lo) return SLOTH_OK;
-    hipLaunchKernelGGL(k_bin, dim3(blocks(it_hi - it_lo, BIN_BLOCK)), dim3(BIN_BLOCK), 0, s, d_draws, d_lo, d_hi,
-                       d_frames, it_lo, it_hi - it_lo, TW, TH, d_cursor.as<uint32_t>(), d_off.as<uint64_t>(),
-                       d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_overflow.as<uint32_t>(), count_only ? 1 : 0);
+    if (count_only)
+        hipLaunchKernelGGL(k_bin<true>, dim3(blocks(it_hi - it_lo, BIN_BLOCK)), dim3(BIN_BLOCK), 0, s, d_draws, d_lo,
+                           d_hi, d_frames, it_lo, it_hi - it_lo, TW, TH, d_cursor.as<uint32_t>(), d_off.as<uint64_t>(),
+                           d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_overflow.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(k_bin<false>, dim3(blocks(it_hi - it_lo, BIN_BLOCK)), dim3(BIN_BLOCK), 0, s, d_draws, d_lo,
+                           d_hi, d_frames, it_lo, it_hi - it_lo, TW, TH, d_cursor.as<uint32_t>(), d_off.as<uint64_t>(),
+                           d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_overflow.as<uint32_t>());
     HIP_TRY(hipGetLastError());
     return SLOTH_OK;
 }
