@@ -238,7 +238,8 @@ struct Job {
         n_z = z;
         n_rows = rows;
         n_items = items;
-        path = (all_even && !force_global()) ? PATH_TILED : PATH_GLOBAL;
+        // the tiled path keeps a draw's shader index in 8 bits of an LDS word
+        path = (all_even && ramps.size() <= 256 && !force_global()) ? PATH_TILED : PATH_GLOBAL;
         int rc;
         if ((rc = d_frames.reserve(sizeof(Frame) * std::max<size_t>(1, frames.size())))) return rc;
         if ((rc = d_draws.reserve(sizeof(Draw) * std::max<size_t>(1, draws.size())))) return rc;
@@ -326,7 +327,7 @@ struct Job {
     int prepare(hipStream_t s)
     {
         if (path == PATH_TILED && !tiled.sized)
-            return tiled.size(frames, draws, d_frames.as<Frame>(), d_draws.as<Draw>(), n_items, s);
+            return tiled.size(frames, draws, d_frames.as<Frame>(), d_draws.as<Draw>(), s);
         return SLOTH_OK;
     }
 

@@ -8,12 +8,16 @@
 //   k_tile_clear   tile cursors <- 0, newline rows <- -1, overflow <- 0,
 //                  image-mode tail cells <- blank
 //   k_bin          one thread per triangle: setup, classification, candidate
-//                  box, newline rows (atomicMax), and the triangle index
-//                  appended to every tile its box reaches, with one atomicAdd
-//                  per distinct tile per wavefront (ballot + prefix count)
-//   k_tile_raster  one workgroup per tile: 64-bit LDS atomicMin of the packed
-//                  keys, a second walk in which each slot's winning fragment
-//                  stages its cell and depth in LDS, then coalesced stores
+//                  box, newline rows (LDS max per block, one atomicMax per
+//                  row), and the triangle index appended to every tile its box
+//                  reaches, with one atomicAdd per distinct tile per wavefront
+//                  (ballot + prefix popcount)
+//   k_tile_raster  one 256-thread workgroup per tile: each list entry's transformed
+//                  triangle goes to an LDS row (found by a hash of its index),
+//                  its fragments do a 64-bit LDS atomicMin of the packed keys,
+//                  then every slot re-evaluates its winner from the LDS row
+//                  and stores its two cells and depths (coalesced).  Lists
+//                  longer than the LDS rows take two walks instead.
 //
 // Tile lists live in one pool at fixed per-tile offsets, sized from an exact
 // counting run of k_bin plus headroom.  If a later run would exceed a tile's
@@ -42,10 +46,14 @@ struct TiledPlan {
     // state.  Pool offsets become invalid until size() runs.
     int plan(std::vector<Frame>& frames, uint64_t rows);
 
+    // Tile size in cells: TW a power of two in 8..128, TH in 4..64.
+    int set_tile(uint32_t tw, uint32_t th);
+    size_t raster_lds() const;
+
     // Counting run over all frames (synchronous): per-tile list lengths ->
     // pool offsets with headroom.
     int size(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
-             const Draw* d_draws, uint64_t n_items, hipStream_t s);
+             const Draw* d_draws, hipStream_t s);
 
     int upload(hipStream_t s) { (void)s; return SLOTH_OK; }
 
@@ -61,11 +69,9 @@ struct TiledPlan {
     static std::vector<const char*> stage_names() { return {"k_tile_clear", "k_bin", "k_tile_raster"}; }
 
   private:
-    int launch_clear(const std::vector<Frame>& frames, const Frame* d_frames, uint32_t f_lo, uint32_t f_hi,
-                     uint32_t* cells, hipStream_t s);
+    int launch_clear(const std::vector<Frame>& frames, uint32_t f_lo, uint32_t f_hi, hipStream_t s);
     int launch_bin(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
-                   const Draw* d_draws, uint32_t f_lo, uint32_t f_hi, uint64_t n_items, bool count_only,
-                   hipStream_t s);
+                   const Draw* d_draws, uint32_t f_lo, uint32_t f_hi, bool count_only, hipStream_t s);
 };
 
 }  // namespace sloth

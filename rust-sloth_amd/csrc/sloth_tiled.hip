@@ -8,7 +8,7 @@
 namespace sloth {
 
 constexpr int BIN_BLOCK = 256;
-constexpr int RASTER_BLOCK = 64;
+constexpr int RASTER_BLOCK = 256;
 
 // ---------------------------------------------------------------------------
 // k_tile_clear
@@ -26,23 +26,10 @@ __global__ void k_tile_clear(uint32_t* __restrict__ cursor, uint64_t n_tiles, in
 // ---------------------------------------------------------------------------
 // k_bin
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t find_draw_t(const Draw* __restrict__ draws, uint32_t d_lo, uint32_t d_hi,
-                                                uint64_t item)
-{
-    uint32_t lo = d_lo, hi = d_hi - 1;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (draws[mid].item_begin <= item) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
 
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane)
-{
-    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
-    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
-    return ((uint64_t)hi << 32) | lo;
-}
+// Pool entries: frame triangle index, plus this bit when the triangle's
+// candidate box is the cheap CLS_BOX one (the raster then skips the proof).
+constexpr uint32_t BOX_FLAG = 0x80000000u;
 
 // Tile rectangles (inclusive, in tile units) that a candidate box reaches:
 // the direct part (2x < W, cell (2x, y)) and the wrapped part (2x >= W, cell
@@ -52,179 +39,184 @@ struct TileRects {
     int wx0, wx1, wy0, wy1;
 };
 
-__device__ __forceinline__ void tile_rects(int bx0, int bx1, int by0, int by1, uint32_t W, uint32_t TW,
-                                           uint32_t TH, TileRects& r)
+__device__ __forceinline__ void tile_rects(int bx0, int bx1, int by0, int by1, uint32_t W, int tws, int ths,
+                                           TileRects& r)
 {
     const int half = (int)(W >> 1);
     int dxe = min(bx1, half - 1);
     if (bx0 <= dxe) {
-        r.dx0 = (2 * bx0) / (int)TW;
-        r.dx1 = (2 * dxe) / (int)TW;
-        r.dy0 = by0 / (int)TH;
-        r.dy1 = by1 / (int)TH;
+        r.dx0 = (2 * bx0) >> tws;
+        r.dx1 = (2 * dxe) >> tws;
+        r.dy0 = by0 >> ths;
+        r.dy1 = by1 >> ths;
     } else {
-        r.dx0 = 1;
-        r.dx1 = 0;
-        r.dy0 = 1;
-        r.dy1 = 0;
+        r.dx0 = r.dy0 = 1;
+        r.dx1 = r.dy1 = 0;
     }
     int wxs = max(bx0, half);
     if (wxs <= bx1) {
-        r.wx0 = (2 * wxs - (int)W) / (int)TW;
-        r.wx1 = (2 * bx1 - (int)W) / (int)TW;
-        r.wy0 = (by0 + 1) / (int)TH;
-        r.wy1 = (by1 + 1) / (int)TH;
+        r.wx0 = (2 * wxs - (int)W) >> tws;
+        r.wx1 = (2 * bx1 - (int)W) >> tws;
+        r.wy0 = (by0 + 1) >> ths;
+        r.wy1 = (by1 + 1) >> ths;
     } else {
-        r.wx0 = 1;
-        r.wx1 = 0;
-        r.wy0 = 1;
-        r.wy1 = 0;
+        r.wx0 = r.wy0 = 1;
+        r.wx1 = r.wy1 = 0;
     }
 }
 
-__device__ __forceinline__ int rect_count(int x0, int x1, int y0, int y1)
+// Walks the lane's tiles: the direct rectangle, then the wrapped one minus the
+// tiles the direct rectangle already holds.  phase 2 = done.
+struct TileIter {
+    int tx, ty, phase;
+};
+
+__device__ __forceinline__ bool in_direct(const TileRects& r, int tx, int ty)
 {
-    return (x0 <= x1 && y0 <= y1) ? (x1 - x0 + 1) * (y1 - y0 + 1) : 0;
+    return tx >= r.dx0 && tx <= r.dx1 && ty >= r.dy0 && ty <= r.dy1;
 }
 
-// k-th tile of the lane's set (direct rectangle first, then the wrapped one);
-// returns false for a wrapped tile that the direct rectangle already holds.
-__device__ __forceinline__ bool tile_at(const TileRects& r, int k, int& tx, int& ty)
+__device__ __forceinline__ void iter_settle(const TileRects& r, TileIter& it)
 {
-    int nd = rect_count(r.dx0, r.dx1, r.dy0, r.dy1);
-    if (k < nd) {
-        int w = r.dx1 - r.dx0 + 1;
-        tx = r.dx0 + k % w;
-        ty = r.dy0 + k / w;
-        return true;
+    for (;;) {
+        if (it.phase == 0) {
+            if (r.dx0 <= r.dx1 && it.ty <= r.dy1) return;
+            it.phase = 1;
+            it.tx = r.wx0;
+            it.ty = r.wy0;
+        }
+        if (it.phase == 1) {
+            if (!(r.wx0 <= r.wx1) || it.ty > r.wy1) {
+                it.phase = 2;
+                return;
+            }
+            if (!in_direct(r, it.tx, it.ty)) return;
+            if (++it.tx > r.wx1) {
+                it.tx = r.wx0;
+                ++it.ty;
+            }
+            continue;
+        }
+        return;
     }
-    k -= nd;
-    int w = r.wx1 - r.wx0 + 1;
-    tx = r.wx0 + k % w;
-    ty = r.wy0 + k / w;
-    return !(nd > 0 && tx >= r.dx0 && tx <= r.dx1 && ty >= r.dy0 && ty <= r.dy1);
 }
 
-// COUNT = true is the sizing run: it only counts each tile's entries.
+__device__ __forceinline__ void iter_next(const TileRects& r, TileIter& it)
+{
+    if (it.phase == 0) {
+        if (++it.tx > r.dx1) {
+            it.tx = r.dx0;
+            ++it.ty;
+        }
+    } else if (it.phase == 1) {
+        if (++it.tx > r.wx1) {
+            it.tx = r.wx0;
+            ++it.ty;
+        }
+    }
+    iter_settle(r, it);
+}
+
+// Grid: x = blocks of one draw's triangles, y = draw index - d_lo, so the draw,
+// its frame and its matrix are uniform per block (scalar loads).
 template <bool COUNT>
-__global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draws, uint32_t d_lo, uint32_t d_hi,
-                                                   const Frame* __restrict__ frames, uint64_t item_base,
-                                                   uint64_t n_items, uint32_t TW, uint32_t TH,
+__global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draws, uint32_t d_lo,
+                                                   const Frame* __restrict__ frames, int tws, int ths,
                                                    uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
                                                    uint32_t* __restrict__ pool, int32_t* __restrict__ nl,
                                                    uint32_t* __restrict__ overflow)
 {
-    const uint64_t t_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = t_id < n_items;
+    __shared__ int32_t s_nl[BIN_BLOCK];
+    __shared__ int s_lo, s_hi;
+
+    const Draw& dr = draws[d_lo + blockIdx.y];
+    const uint32_t base = blockIdx.x * BIN_BLOCK;
+    if (base >= dr.n) return;
+    const uint32_t local = base + threadIdx.x;
+    const bool valid = local < dr.n;
+    const uint32_t frame = dr.frame;
+    const Frame& fr = frames[frame];
+    const uint32_t W = fr.W, H = fr.H;
+    const uint32_t tri = dr.tri_begin + local;
     const int lane = (int)__lane_id();
 
-    uint32_t tri = 0, frame = 0;
-    uint64_t tile_begin = 0;
-    uint32_t tiles_x = 0;
-    int nt = 0;
-    TileRects r;
-    r.dx0 = r.wx0 = 1;
-    r.dx1 = r.wx1 = 0;
-    r.dy0 = r.wy0 = 1;
-    r.dy1 = r.wy1 = 0;
-
-    // Blocks rarely straddle two draws: if this one does not, the draw index is
-    // uniform and its fields (the 4x4 matrix included) come in scalar loads.
-    const uint64_t blk0 = item_base + (uint64_t)blockIdx.x * blockDim.x;
-    const uint64_t blk1 = item_base + min((uint64_t)blockIdx.x * blockDim.x + blockDim.x, n_items) - 1;
-    const uint32_t d_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)find_draw_t(draws, d_lo, d_hi, blk0));
-    const uint32_t d_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)find_draw_t(draws, d_lo, d_hi, blk1));
     int y0 = 0, y1d = 0;
+    uint32_t entry = tri;
+    TileRects r;
+    r.dx0 = r.dy0 = r.wx0 = r.wy0 = 1;
+    r.dx1 = r.dy1 = r.wx1 = r.wy1 = 0;
     if (valid) {
-        uint64_t item = item_base + t_id;
-        uint32_t d = d_first == d_last ? d_first : find_draw_t(draws, d_lo, d_hi, item);
-        const Draw& dr = draws[d];
-        uint32_t local = (uint32_t)(item - dr.item_begin);
-        frame = dr.frame;
-        const Frame& fr = frames[frame];
         Tri t;
-        setup_tri(dr.M, dr.xyz + (size_t)local * 9, fr.W, fr.H, t);
-        tri = dr.tri_begin + local;
-        if (fr.image) {
-            y0 = t.y0;
-            y1d = t.y1d;
-        }
+        setup_tri(dr.M, dr.xyz + (size_t)local * 9, W, H, t);
+        y0 = t.y0;
+        y1d = t.y1d;
         uint32_t cls = setup_class(t);
+        if (cls == CLS_BOX) entry |= BOX_FLAG;
         int bx0, bx1, by0, by1;
-        if (tri_box(t, cls, bx0, bx1, by0, by1)) {
-            tile_rects(bx0, bx1, by0, by1, fr.W, TW, TH, r);
-            nt = rect_count(r.dx0, r.dx1, r.dy0, r.dy1) + rect_count(r.wx0, r.wx1, r.wy0, r.wy1);
-        }
-        tile_begin = fr.tile_begin;
-        tiles_x = fr.tiles_x;
+        if (tri_box(t, cls, bx0, bx1, by0, by1)) tile_rects(bx0, bx1, by0, by1, W, tws, ths, r);
     }
 
     // rasterizer.rs:89-91: a newline cell after every row of the domain; the
-    // last triangle (in draw order) whose rows include r decides row r.  When
-    // the wave is in one frame and its rows span < 64, one ballot per row finds
-    // the highest lane (= latest triangle) covering it and one atomicMax
-    // publishes it; otherwise every lane publishes its own rows.
-    {
+    // last triangle (in draw order) whose rows include r decides row r.  The
+    // block reduces its rows in LDS and publishes one atomicMax per row.
+    if (fr.image) {
+        if (threadIdx.x == 0) {
+            s_lo = 1 << 30;
+            s_hi = -1;
+        }
+        s_nl[threadIdx.x] = -1;
+        __syncthreads();
         const bool has_rows = valid && y0 < y1d;
-        const uint64_t rmask = __ballot(has_rows);
-        if (rmask) {
-            const int l0 = __ffsll((unsigned long long)rmask) - 1;
-            const uint32_t f0 = (uint32_t)__builtin_amdgcn_readlane((int)frame, l0);
-            const bool same = __ballot(has_rows && frame != f0) == 0;
-            int lo_r = has_rows ? y0 : (1 << 30), hi_r = has_rows ? y1d - 1 : -1;
-            for (int sh = 32; sh >= 1; sh >>= 1) {
-                lo_r = min(lo_r, __shfl_xor(lo_r, sh));
-                hi_r = max(hi_r, __shfl_xor(hi_r, sh));
-            }
-            lo_r = __builtin_amdgcn_readfirstlane(lo_r);
-            hi_r = __builtin_amdgcn_readfirstlane(hi_r);
-            int32_t* fnl = nl + frames[f0].nl_off;
-            if (same && hi_r - lo_r < 64) {
-                for (int row = lo_r; row <= hi_r; ++row) {
-                    uint64_t m = __ballot(has_rows && y0 <= row && row < y1d);
-                    if (m) {
-                        int top = 63 - __clzll((long long)m);
-                        if (lane == top) atomicMax(&fnl[row], (int32_t)tri);
-                    }
-                }
+        if (has_rows) {
+            atomicMin(&s_lo, y0);
+            atomicMax(&s_hi, y1d - 1);
+        }
+        __syncthreads();
+        const int lo = s_lo, hi = s_hi;
+        int32_t* fnl = nl + fr.nl_off;
+        if (hi >= lo) {
+            if (hi - lo < BIN_BLOCK) {
+                if (has_rows)
+                    for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)tri);
+                __syncthreads();
+                const int32_t v = s_nl[threadIdx.x];
+                if ((int)threadIdx.x <= hi - lo && v >= 0) atomicMax(&fnl[lo + (int)threadIdx.x], v);
             } else if (has_rows) {
-                int32_t* mine = nl + frames[frame].nl_off;
-                for (int y = y0; y < y1d; ++y) atomicMax(&mine[y], (int32_t)tri);
+                for (int y = y0; y < y1d; ++y) atomicMax(&fnl[y], (int32_t)tri);
             }
         }
     }
 
-    // Append `tri` to each of its tiles; lanes that share a tile share one
-    // atomicAdd on that tile's cursor.
-    int k = 0;
+    // Append the entry to each of its tiles; lanes that share a tile share one
+    // atomicAdd on that tile's cursor (ballot + prefix popcount).
+    const uint64_t tile_begin = fr.tile_begin;
+    const uint32_t tiles_x = fr.tiles_x;
+    TileIter it;
+    it.phase = 0;
+    it.tx = r.dx0;
+    it.ty = r.dy0;
+    iter_settle(r, it);
     for (;;) {
-        int tx = 0, ty = 0;
-        bool active = false;
-        while (k < nt) {
-            if (tile_at(r, k, tx, ty)) {
-                active = true;
-                break;
-            }
-            ++k;
-        }
-        uint64_t amask = __ballot(active);
+        const bool active = it.phase < 2;
+        const uint64_t amask = __ballot(active);
         if (amask == 0) break;
-        uint64_t key = active ? tile_begin + (uint64_t)ty * tiles_x + (uint64_t)tx : ~0ull;
-        int leader = __ffsll((unsigned long long)amask) - 1;
-        uint64_t lk = readlane64(key, leader);
-        bool match = active && key == lk;
-        uint64_t mmask = __ballot(match);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&cursor[lk], (uint32_t)__popcll(mmask));
-        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        const uint32_t key = active ? (uint32_t)it.ty * tiles_x + (uint32_t)it.tx : 0xFFFFFFFFu;
+        const int leader = __ffsll((unsigned long long)amask) - 1;
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const bool match = active && key == lk;
+        const uint64_t mmask = __ballot(match);
+        const uint64_t gt = tile_begin + lk;
+        uint32_t pos0 = 0;
+        if (lane == leader) pos0 = atomicAdd(&cursor[gt], (uint32_t)__popcll(mmask));
+        pos0 = (uint32_t)__builtin_amdgcn_readlane((int)pos0, leader);
         if (match) {
-            uint32_t pos = base + (uint32_t)__popcll(mmask & __lanemask_lt());
+            const uint32_t pos = pos0 + (uint32_t)__popcll(mmask & __lanemask_lt());
             if (!COUNT) {
-                uint64_t o = off[lk], cap = off[lk + 1] - o;
-                if (pos < cap) pool[o + pos] = tri;
+                const uint64_t o = off[gt], cap = off[gt + 1] - o;
+                if (pos < cap) pool[o + pos] = entry;
                 else overflow[frame] = 1u;
             }
-            ++k;
+            iter_next(r, it);
         }
     }
 }
@@ -232,6 +224,11 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 // ---------------------------------------------------------------------------
 // k_tile_raster
 // ---------------------------------------------------------------------------
+constexpr uint32_t CACHE_CAP = 256;   // list length up to which the fast path runs
+constexpr uint32_t HASH_CAP = 512;    // open-addressing table: triangle -> cache row
+constexpr uint32_t CACHE_WORDS = 12;
+constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
+
 __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws, const Frame& fr, uint32_t tri)
 {
     uint32_t lo = fr.draw_begin, hi = fr.draw_end - 1;
@@ -242,9 +239,11 @@ __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws,
     return draws[lo];
 }
 
+__device__ __forceinline__ uint32_t hash_tri(uint32_t T) { return (T * 0x9E3779B1u) >> 23; }   // 9 bits
+
 struct TileGeom {
     int tc0, tc1, tr0, tr1;   // cell columns [tc0,tc1), rows [tr0,tr1)
-    int sw;                   // slots per tile row (TW/2)
+    int sws;                  // log2(slots per tile row) = log2(TW/2)
     int W;
 };
 
@@ -257,27 +256,39 @@ __device__ __forceinline__ void walk_tile(const TileGeom& g, int bx0, int bx1, i
     int xa = max(bx0, g.tc0 >> 1), xb = min(bx1, (g.tc1 >> 1) - 1);
     int ya = max(by0, g.tr0), yb = min(by1, g.tr1 - 1);
     for (int y = ya; y <= yb; ++y)
-        for (int x = xa; x <= xb; ++x) f(x, y, (y - g.tr0) * g.sw + (x - (g.tc0 >> 1)), 1u);
+        for (int x = xa; x <= xb; ++x) f(x, y, ((y - g.tr0) << g.sws) + (x - (g.tc0 >> 1)), 1u);
     int wx0 = (g.tc0 + g.W) >> 1;
     xa = max(bx0, wx0);
     xb = min(bx1, ((g.tc1 + g.W) >> 1) - 1);
     ya = max(by0, g.tr0 - 1);
     yb = min(by1, g.tr1 - 2);
     for (int y = ya; y <= yb; ++y)
-        for (int x = xa; x <= xb; ++x) f(x, y, (y + 1 - g.tr0) * g.sw + (x - wx0), 0u);
+        for (int x = xa; x <= xb; ++x) f(x, y, ((y + 1 - g.tr0) << g.sws) + (x - wx0), 0u);
+}
+
+// Candidate box of a pool entry.
+__device__ __forceinline__ bool entry_box(const Tri& t, uint32_t e, int& bx0, int& bx1, int& by0, int& by1)
+{
+    if (e & BOX_FLAG) {
+        bx0 = max(t.x0, (int)floorf(t.minx));
+        bx1 = min(t.x1d - 1, (int)ceilf(t.maxx));
+        by0 = t.y0;
+        by1 = t.y1d - 1;
+        return bx0 <= bx1 && by0 <= by1;
+    }
+    return tri_box(t, setup_class(t), bx0, bx1, by0, by1);
 }
 
 __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
     const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
-    uint32_t f_hi, uint64_t tile_base, uint32_t TW, uint32_t TH, const uint32_t* __restrict__ cursor,
+    uint32_t f_hi, uint64_t tile_base, int tws, int ths, const uint32_t* __restrict__ cursor,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ pool, const int32_t* __restrict__ nl,
     uint32_t* __restrict__ cells, float* __restrict__ zbuf, int32_t* __restrict__ win)
 {
     extern __shared__ uint64_t smem[];
-    const uint32_t S = (TW >> 1) * TH;
+    const int sws = tws - 1;
+    const uint32_t S = 1u << (sws + ths);
     uint64_t* keys = smem;
-    uint32_t* st_cell = (uint32_t*)(smem + S);
-    float* st_z = (float*)(st_cell + S);
 
     const uint64_t gt = tile_base + blockIdx.x;
     uint32_t lo = f_lo, hi = f_hi - 1;
@@ -288,27 +299,22 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
     const Frame& fr = frames[lo];
     const uint32_t W = fr.W, H = fr.H;
     const uint32_t local = (uint32_t)(gt - fr.tile_begin);
-    const uint32_t tx = local % fr.tiles_x, ty = local / fr.tiles_x;
+    const uint32_t ty = local / fr.tiles_x, tx = local - ty * fr.tiles_x;
     TileGeom g;
-    g.tc0 = (int)(tx * TW);
-    g.tc1 = min((int)W, g.tc0 + (int)TW);
-    g.tr0 = (int)(ty * TH);
-    g.tr1 = min((int)H, g.tr0 + (int)TH);
-    g.sw = (int)(TW >> 1);
+    g.tc0 = (int)(tx << tws);
+    g.tc1 = min((int)W, g.tc0 + (1 << tws));
+    g.tr0 = (int)(ty << ths);
+    g.tr1 = min((int)H, g.tr0 + (1 << ths));
+    g.sws = sws;
     g.W = (int)W;
 
-    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
     const uint64_t o = off[gt];
     const uint32_t n = min(cursor[gt], (uint32_t)(off[gt + 1] - o));
     const uint32_t* list = pool + o;
-    __syncthreads();
-
     const bool one_draw = fr.draw_end - fr.draw_begin == 1;
     auto draw_for = [&](uint32_t T) -> const Draw& {
         return one_draw ? draws[fr.draw_begin] : draw_of_t(draws, fr, T);
     };
-
-    // pass 1: depth test of every candidate fragment
     auto test = [&](const Tri& t, uint32_t T, int bx0, int bx1, int by0, int by1) {
         walk_tile(g, bx0, bx1, by0, by1, [&](int x, int y, int slot, uint32_t sub) {
             float w0, w1, w2;
@@ -319,92 +325,152 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
             atomicMin((unsigned long long*)&keys[slot], (unsigned long long)make_key(z, T, sub));
         });
     };
-    // pass 2: the fragment whose key won stages its cell and depth
-    auto stage = [&](const Tri& t, uint32_t T, const Draw& dr, int bx0, int bx1, int by0, int by1) {
-        const uint32_t li = T - dr.tri_begin;
-        bool have_nz = false;
-        float nzu = 0.0f;
-        uint32_t color = 0;
-        walk_tile(g, bx0, bx1, by0, by1, [&](int x, int y, int slot, uint32_t sub) {
-            float w0, w1, w2;
-            edges(t, (float)x, (float)y, w0, w1, w2);
-            if (!inside(w0, w1, w2)) return;
-            float z = depth(t, w1, w2);
-            if (!(z < F32_MAX)) return;
-            if (keys[slot] != make_key(z, T, sub)) return;
-            if (!have_nz) {
-                nzu = unit_normal_z(t);
-                const uint8_t* c = dr.rgb + (size_t)li * 3;
-                color = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24);
-                have_nz = true;
-            }
-            float s = shade_value(nzu, t.a, w0, w1, w2);
-            st_cell[slot] = (uint32_t)ramp_glyph(ramps[dr.ramp], s) | color;
-            st_z[slot] = z;
-        });
-    };
 
-    // The first entry of each thread stays in registers for both passes.
-    Tri t0;
-    uint32_t T0 = 0;
-    int c0x0 = 0, c0x1 = -1, c0y0 = 0, c0y1 = -1;
-    bool ok0 = false;
-    if (threadIdx.x < n) {
-        T0 = list[threadIdx.x];
-        const Draw& dr = draw_for(T0);
-        setup_tri(dr.M, dr.xyz + (size_t)(T0 - dr.tri_begin) * 9, W, H, t0);
-        ok0 = tri_box(t0, setup_class(t0), c0x0, c0x1, c0y0, c0y1);
-    }
-    if (ok0) test(t0, T0, c0x0, c0x1, c0y0, c0y1);
-    for (uint32_t e = threadIdx.x + blockDim.x; e < n; e += blockDim.x) {
-        const uint32_t T = list[e];
-        const Draw& dr = draw_for(T);
-        Tri t;
-        setup_tri(dr.M, dr.xyz + (size_t)(T - dr.tri_begin) * 9, W, H, t);
-        int bx0, bx1, by0, by1;
-        if (tri_box(t, setup_class(t), bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
-    }
-    __syncthreads();
-    if (ok0) stage(t0, T0, draw_for(T0), c0x0, c0x1, c0y0, c0y1);
-    for (uint32_t e = threadIdx.x + blockDim.x; e < n; e += blockDim.x) {
-        const uint32_t T = list[e];
-        const Draw& dr = draw_for(T);
-        Tri t;
-        setup_tri(dr.M, dr.xyz + (size_t)(T - dr.tri_begin) * 9, W, H, t);
-        int bx0, bx1, by0, by1;
-        if (tri_box(t, setup_class(t), bx0, bx1, by0, by1)) stage(t, T, dr, bx0, bx1, by0, by1);
-    }
-    __syncthreads();
-
-    // write-out: every slot owns cells (id, id+1) and z slots (id, id+1)
     const int32_t* fnl = nl + fr.nl_off;
     uint32_t* fc = cells + fr.cell_off;
     float* fz = zbuf + fr.z_off;
-    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
-        int sr = (int)s / g.sw, sc = (int)s - sr * g.sw;
-        int row = g.tr0 + sr, col = g.tc0 + 2 * sc;
-        if (row >= g.tr1 || col >= g.tc1) continue;
-        uint64_t k = keys[s];
+    // Stores the cells and depths of slot s from its winning cell (or blank).
+    auto store = [&](uint32_t s, uint64_t k, uint32_t cell, float z) {
+        const int sr = (int)(s >> sws), sc = (int)(s & ((1u << sws) - 1));
+        const int row = g.tr0 + sr, col = g.tc0 + 2 * sc;
         uint32_t c0 = CELL_BLANK, c1 = CELL_BLANK;
         float z0 = F32_MAX;
         int32_t wt = -1;
         if (k != KEY_EMPTY) {
-            c0 = c1 = st_cell[s];
-            z0 = st_z[s];
+            c0 = c1 = cell;
+            z0 = z;
             wt = (int32_t)((uint32_t)k >> 1);
         }
         if (fr.image && col == 0) {
-            // cell row*W + 1: the newline of row `row` is written after every
-            // pixel of its triangle's row and before its next row, so it
-            // overwrites a wrapped pixel (drawn in row-1) of the same or an
-            // earlier triangle
-            int32_t nlr = fnl[row];
+            // cell row*W + 1: the newline of row `row` comes after every pixel
+            // of its triangle's row and before its next row, so it overwrites
+            // a wrapped pixel (drawn in row-1) of the same or an earlier
+            // triangle
+            const int32_t nlr = fnl[row];
             if (nlr >= 0 && (k == KEY_EMPTY || nlr >= wt)) c1 = CELL_NEWLINE;
         }
-        uint64_t id = (uint64_t)row * W + (uint64_t)col;
+        const uint64_t id = (uint64_t)row * W + (uint64_t)col;
         *(uint2*)(fc + id) = make_uint2(c0, c1);
         *(float2*)(fz + id) = make_float2(z0, F32_MAX);
         if (win) *(int2*)(win + fr.z_off + id) = make_int2(wt, -1);
+    };
+    auto slot_valid = [&](uint32_t s) {
+        return g.tr0 + (int)(s >> sws) < g.tr1 && g.tc0 + 2 * (int)(s & ((1u << sws) - 1)) < g.tc1;
+    };
+
+    if (n <= CACHE_CAP) {
+        // Fast path: one walk.  Each entry's transformed triangle goes to an
+        // LDS row found through a hash of its triangle index; the winner of a
+        // slot is re-evaluated from that row.
+        uint32_t* htag = (uint32_t*)(keys + S);
+        uint32_t* hrow = htag + HASH_CAP;
+        float* cache = (float*)(hrow + HASH_CAP);
+        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
+        for (uint32_t h = threadIdx.x; h < HASH_CAP; h += blockDim.x) htag[h] = EMPTY32;
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
+            const uint32_t e = list[p];
+            const uint32_t T = e & ~BOX_FLAG;
+            const Draw& dr = draw_for(T);
+            const uint32_t li = T - dr.tri_begin;
+            Tri t;
+            setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
+            const float na = unit_normal_z(t) * t.a;   // rasterizer.rs:76 multiplies these first
+            const uint8_t* c = dr.rgb + (size_t)li * 3;
+            const uint32_t cw = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24) | dr.ramp;
+            float4* row4 = (float4*)(cache + p * CACHE_WORDS);
+            row4[0] = make_float4(t.x1, t.y1, t.x2, t.y2);
+            row4[1] = make_float4(t.x3, t.y3, t.z1, t.z2 - t.z1);
+            row4[2] = make_float4(t.z3 - t.z1, t.a, na, __uint_as_float(cw));
+            uint32_t h = hash_tri(T);
+            for (uint32_t probe = 0; probe < HASH_CAP; ++probe) {
+                if (atomicCAS(&htag[h], EMPTY32, T) == EMPTY32) {
+                    hrow[h] = p;
+                    break;
+                }
+                h = (h + 1) & (HASH_CAP - 1);
+            }
+            int bx0, bx1, by0, by1;
+            if (entry_box(t, e, bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
+        }
+        __syncthreads();
+        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+            if (!slot_valid(s)) continue;
+            const uint64_t k = keys[s];
+            uint32_t cell = CELL_BLANK;
+            float z = F32_MAX;
+            if (k != KEY_EMPTY) {
+                const uint32_t T = (uint32_t)k >> 1, sub = (uint32_t)k & 1u;
+                uint32_t h = hash_tri(T);
+                for (uint32_t probe = 0; probe < HASH_CAP && htag[h] != T; ++probe) h = (h + 1) & (HASH_CAP - 1);
+                const float4* row4 = (const float4*)(cache + hrow[h] * CACHE_WORDS);
+                const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
+                const int row = g.tr0 + (int)(s >> sws), col = g.tc0 + 2 * (int)(s & ((1u << sws) - 1));
+                const float px = (float)(sub ? (col >> 1) : ((col + g.W) >> 1));
+                const float py = (float)(sub ? row : row - 1);
+                // the same expressions as edges(), depth() and shade_value()
+                const float x1 = q0.x, y1 = q0.y, x2 = q0.z, y2 = q0.w, x3 = q1.x, y3 = q1.y;
+                const float w0 = (x3 - x2) * (py - y2) - (y3 - y2) * (px - x2);
+                const float w1 = (x1 - x3) * (py - y3) - (y1 - y3) * (px - x3);
+                const float w2 = (x2 - x1) * (py - y1) - (y2 - y1) * (px - x1);
+                z = q1.z + q2.y * (w1 * q1.w + w2 * q2.x);
+                const float sh = q2.z * (w0 + w1 + w2);
+                const uint32_t cw = __float_as_uint(q2.w);
+                cell = (uint32_t)ramp_glyph(ramps[cw & 0xFFu], sh) | (cw & 0xFFFFFF00u);
+            }
+            store(s, k, cell, z);
+        }
+    } else {
+        // Long lists: two walks; the second stages the winner's cell and depth.
+        uint32_t* st_cell = (uint32_t*)(keys + S);
+        float* st_z = (float*)(st_cell + S);
+        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
+            const uint32_t e = list[p];
+            const uint32_t T = e & ~BOX_FLAG;
+            const Draw& dr = draw_for(T);
+            Tri t;
+            setup_tri(dr.M, dr.xyz + (size_t)(T - dr.tri_begin) * 9, W, H, t);
+            int bx0, bx1, by0, by1;
+            if (entry_box(t, e, bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
+        }
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
+            const uint32_t e = list[p];
+            const uint32_t T = e & ~BOX_FLAG;
+            const Draw& dr = draw_for(T);
+            const uint32_t li = T - dr.tri_begin;
+            Tri t;
+            setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
+            int bx0, bx1, by0, by1;
+            if (!entry_box(t, e, bx0, bx1, by0, by1)) continue;
+            bool have_nz = false;
+            float na = 0.0f;
+            uint32_t color = 0;
+            walk_tile(g, bx0, bx1, by0, by1, [&](int x, int y, int slot, uint32_t sub) {
+                float w0, w1, w2;
+                edges(t, (float)x, (float)y, w0, w1, w2);
+                if (!inside(w0, w1, w2)) return;
+                float z = depth(t, w1, w2);
+                if (!(z < F32_MAX)) return;
+                if (keys[slot] != make_key(z, T, sub)) return;
+                if (!have_nz) {
+                    na = unit_normal_z(t) * t.a;
+                    const uint8_t* c = dr.rgb + (size_t)li * 3;
+                    color = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24);
+                    have_nz = true;
+                }
+                st_cell[slot] = (uint32_t)ramp_glyph(ramps[dr.ramp], na * (w0 + w1 + w2)) | color;
+                st_z[slot] = z;
+            });
+        }
+        __syncthreads();
+        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+            if (!slot_valid(s)) continue;
+            const uint64_t k = keys[s];
+            store(s, k, k != KEY_EMPTY ? st_cell[s] : CELL_BLANK, k != KEY_EMPTY ? st_z[s] : F32_MAX);
+        }
     }
     // image mode: the H cells after the W*H grid are never written (context.rs:38-40)
     if (fr.image && local == 0)
@@ -415,6 +481,30 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
 // TiledPlan
 // ---------------------------------------------------------------------------
 static unsigned blocks(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
+
+static int log2u(uint32_t v)
+{
+    int s = 0;
+    while ((1u << s) < v) ++s;
+    return s;
+}
+
+int TiledPlan::set_tile(uint32_t tw, uint32_t th)
+{
+    if (tw < 8 || tw > 128 || th < 4 || th > 64 || (tw & (tw - 1)) || (th & (th - 1)))
+        return fail(SLOTH_EINVAL, "tile size %ux%u: width must be a power of two in 8..128, height in 4..64", tw, th);
+    if (tw != TW || th != TH) sized = false;
+    TW = tw;
+    TH = th;
+    return SLOTH_OK;
+}
+
+size_t TiledPlan::raster_lds() const
+{
+    size_t S = (size_t)(TW / 2) * TH;
+    size_t fast = (size_t)HASH_CAP * 8 + (size_t)CACHE_CAP * CACHE_WORDS * 4;
+    return S * 8 + std::max(fast, S * 8);
+}
 
 int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
 {
@@ -436,11 +526,8 @@ int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
     return SLOTH_OK;
 }
 
-int TiledPlan::launch_clear(const std::vector<Frame>& frames, const Frame* d_frames, uint32_t f_lo, uint32_t f_hi,
-                            uint32_t* cells, hipStream_t s)
+int TiledPlan::launch_clear(const std::vector<Frame>& frames, uint32_t f_lo, uint32_t f_hi, hipStream_t s)
 {
-    (void)d_frames;
-    (void)cells;
     const Frame& a = frames[f_lo];
     const Frame& b = frames[f_hi - 1];
     uint64_t t_lo = a.tile_begin, t_hi = b.tile_begin + (uint64_t)b.tiles_x * b.tiles_y;
@@ -454,33 +541,37 @@ int TiledPlan::launch_clear(const std::vector<Frame>& frames, const Frame* d_fra
 }
 
 int TiledPlan::launch_bin(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
-                          const Draw* d_draws, uint32_t f_lo, uint32_t f_hi, uint64_t n_items, bool count_only,
-                          hipStream_t s)
+                          const Draw* d_draws, uint32_t f_lo, uint32_t f_hi, bool count_only, hipStream_t s)
 {
-    uint32_t d_lo = frames[f_lo].draw_begin, d_hi = frames[f_hi - 1].draw_end;
-    uint64_t it_lo = d_lo < draws.size() ? draws[d_lo].item_begin : n_items;
-    uint64_t it_hi = d_hi < draws.size() ? draws[d_hi].item_begin : n_items;
-    if (it_hi <= it_lo) return SLOTH_OK;
-    if (count_only)
-        hipLaunchKernelGGL(k_bin<true>, dim3(blocks(it_hi - it_lo, BIN_BLOCK)), dim3(BIN_BLOCK), 0, s, d_draws, d_lo,
-                           d_hi, d_frames, it_lo, it_hi - it_lo, TW, TH, d_cursor.as<uint32_t>(), d_off.as<uint64_t>(),
-                           d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_overflow.as<uint32_t>());
-    else
-        hipLaunchKernelGGL(k_bin<false>, dim3(blocks(it_hi - it_lo, BIN_BLOCK)), dim3(BIN_BLOCK), 0, s, d_draws, d_lo,
-                           d_hi, d_frames, it_lo, it_hi - it_lo, TW, TH, d_cursor.as<uint32_t>(), d_off.as<uint64_t>(),
-                           d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_overflow.as<uint32_t>());
-    HIP_TRY(hipGetLastError());
+    const uint32_t d_lo = frames[f_lo].draw_begin, d_hi = frames[f_hi - 1].draw_end;
+    const int tws = log2u(TW), ths = log2u(TH);
+    for (uint32_t d0 = d_lo; d0 < d_hi; d0 += 65535u) {
+        const uint32_t d1 = std::min(d_hi, d0 + 65535u);
+        uint32_t max_n = 0;
+        for (uint32_t d = d0; d < d1; ++d) max_n = std::max(max_n, draws[d].n);
+        if (max_n == 0) continue;
+        dim3 grid(blocks(max_n, BIN_BLOCK), d1 - d0);
+        if (count_only)
+            hipLaunchKernelGGL(k_bin<true>, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
+                               d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
+                               d_nl.as<int32_t>(), d_overflow.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(k_bin<false>, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
+                               d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
+                               d_nl.as<int32_t>(), d_overflow.as<uint32_t>());
+        HIP_TRY(hipGetLastError());
+    }
     return SLOTH_OK;
 }
 
 int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
-                    const Draw* d_draws, uint64_t n_items, hipStream_t s)
+                    const Draw* d_draws, hipStream_t s)
 {
     if (frames.empty()) return SLOTH_OK;
     const uint32_t F = (uint32_t)frames.size();
     int rc;
-    if ((rc = launch_clear(frames, d_frames, 0, F, nullptr, s))) return rc;
-    if ((rc = launch_bin(frames, draws, d_frames, d_draws, 0, F, n_items, true, s))) return rc;
+    if ((rc = launch_clear(frames, 0, F, s))) return rc;
+    if ((rc = launch_bin(frames, draws, d_frames, d_draws, 0, F, true, s))) return rc;
     std::vector<uint32_t> counts(n_tiles);
     if (n_tiles)
         HIP_TRY(hipMemcpyAsync(counts.data(), d_cursor.p, sizeof(uint32_t) * n_tiles, hipMemcpyDeviceToHost, s));
@@ -504,21 +595,19 @@ int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>&
                       float* z, int32_t* win, hipStream_t s, std::vector<hipEvent_t>* ev)
 {
     if (!sized) return fail(SLOTH_ESTATE, "tile lists not sized");
-    uint64_t n_items = draws.empty() ? 0 : draws.back().item_begin + draws.back().n;
     int rc;
     if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
-    if ((rc = launch_clear(frames, d_frames, f_lo, f_hi, cells, s))) return rc;
+    if ((rc = launch_clear(frames, f_lo, f_hi, s))) return rc;
     if (ev) HIP_TRY(hipEventRecord((*ev)[1], s));
-    if ((rc = launch_bin(frames, draws, d_frames, d_draws, f_lo, f_hi, n_items, false, s))) return rc;
+    if ((rc = launch_bin(frames, draws, d_frames, d_draws, f_lo, f_hi, false, s))) return rc;
     if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
     const Frame& a = frames[f_lo];
     const Frame& b = frames[f_hi - 1];
     uint64_t t_lo = a.tile_begin, t_hi = b.tile_begin + (uint64_t)b.tiles_x * b.tiles_y;
-    size_t lds = (size_t)(TW / 2) * TH * 16;
     if (t_hi > t_lo) {
-        hipLaunchKernelGGL(k_tile_raster, dim3((unsigned)(t_hi - t_lo)), dim3(RASTER_BLOCK), lds, s, d_draws, d_ramps,
-                           d_frames, f_lo, f_hi, t_lo, TW, TH, d_cursor.as<uint32_t>(), d_off.as<uint64_t>(),
-                           d_pool.as<uint32_t>(), d_nl.as<int32_t>(), cells, z, win);
+        hipLaunchKernelGGL(k_tile_raster, dim3((unsigned)(t_hi - t_lo)), dim3(RASTER_BLOCK), raster_lds(), s, d_draws,
+                           d_ramps, d_frames, f_lo, f_hi, t_lo, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
+                           d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(), cells, z, win);
         HIP_TRY(hipGetLastError());
     }
     if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
