@@ -41,6 +41,10 @@ struct TiledPlan {
     uint64_t generation = 0;         // bumped whenever size() runs (device pointers may change)
     std::vector<uint64_t> off;       // host copy of the per-tile pool offsets (n_tiles + 1)
     DevBuf d_cursor, d_off, d_pool, d_nl, d_overflow;
+    // triangles whose candidate box k_careful computes: (draw, local index)
+    // in, (triangle, box) out; capacity from the counting run plus headroom
+    DevBuf d_careful_cnt, d_careful_in, d_careful_box;
+    uint32_t careful_cap = 0;
 
     // Sets the tile grid of every frame and allocates per-tile and per-row
     // state.  Pool offsets become invalid until size() runs.
@@ -57,8 +61,8 @@ struct TiledPlan {
 
     int upload(hipStream_t s) { (void)s; return SLOTH_OK; }
 
-    // clear + bin + raster over frames [f_lo, f_hi).  ev (optional) gets 4
-    // events recorded around the 3 stages.
+    // clear + bin + careful + raster over frames [f_lo, f_hi).  ev (optional)
+    // gets 5 events recorded around the 4 stages.
     int launch(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
                const Draw* d_draws, const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells, float* z,
                int32_t* win, hipStream_t s, std::vector<hipEvent_t>* ev);
@@ -66,12 +70,13 @@ struct TiledPlan {
     // Reads the overflow flags of frames [0, nf) (synchronises the stream).
     int overflowed(uint32_t nf, hipStream_t s, bool& ovf);
 
-    static std::vector<const char*> stage_names() { return {"k_tile_clear", "k_bin", "k_tile_raster"}; }
+    static std::vector<const char*> stage_names() { return {"k_tile_clear", "k_bin", "k_careful", "k_tile_raster"}; }
 
   private:
     int launch_clear(const std::vector<Frame>& frames, uint32_t f_lo, uint32_t f_hi, hipStream_t s);
     int launch_bin(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
                    const Draw* d_draws, uint32_t f_lo, uint32_t f_hi, bool count_only, hipStream_t s);
+    int launch_careful(const Frame* d_frames, const Draw* d_draws, bool count_only, hipStream_t s);
 };
 
 }  // namespace sloth

@@ -14,22 +14,26 @@ constexpr int RASTER_BLOCK = 256;
 // k_tile_clear
 // ---------------------------------------------------------------------------
 __global__ void k_tile_clear(uint32_t* __restrict__ cursor, uint64_t n_tiles, int32_t* __restrict__ nl,
-                             uint64_t n_rows, uint32_t* __restrict__ overflow, uint32_t n_frames)
+                             uint64_t n_rows, uint32_t* __restrict__ overflow, uint32_t n_frames,
+                             uint32_t* __restrict__ careful_cnt)
 {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (uint64_t i = i0; i < n_tiles; i += stride) cursor[i] = 0;
     for (uint64_t i = i0; i < n_rows; i += stride) nl[i] = -1;
     for (uint64_t i = i0; i < n_frames; i += stride) overflow[i] = 0;
+    if (i0 == 0) *careful_cnt = 0;
 }
 
 // ---------------------------------------------------------------------------
-// k_bin
+// tile geometry shared by the binning kernels
 // ---------------------------------------------------------------------------
 
-// Pool entries: frame triangle index, plus this bit when the triangle's
-// candidate box is the cheap CLS_BOX one (the raster then skips the proof).
-constexpr uint32_t BOX_FLAG = 0x80000000u;
+// Pool entries: the frame triangle index of a CLS_BOX triangle (its box is
+// recomputed from floor/ceil of its x extent), or CAREFUL_FLAG | row of the
+// careful-box table for triangles whose box k_careful computed.
+constexpr uint32_t CAREFUL_FLAG = 0x80000000u;
+constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
 
 // Tile rectangles (inclusive, in tile units) that a candidate box reaches:
 // the direct part (2x < W, cell (2x, y)) and the wrapped part (2x >= W, cell
@@ -39,33 +43,34 @@ struct TileRects {
     int wx0, wx1, wy0, wy1;
 };
 
+__device__ __forceinline__ void empty_rects(TileRects& r)
+{
+    r.dx0 = r.dy0 = r.wx0 = r.wy0 = 1;
+    r.dx1 = r.dy1 = r.wx1 = r.wy1 = 0;
+}
+
 __device__ __forceinline__ void tile_rects(int bx0, int bx1, int by0, int by1, uint32_t W, int tws, int ths,
                                            TileRects& r)
 {
     const int half = (int)(W >> 1);
+    empty_rects(r);
     int dxe = min(bx1, half - 1);
-    if (bx0 <= dxe) {
+    if (bx0 <= dxe && by0 <= by1) {
         r.dx0 = (2 * bx0) >> tws;
         r.dx1 = (2 * dxe) >> tws;
         r.dy0 = by0 >> ths;
         r.dy1 = by1 >> ths;
-    } else {
-        r.dx0 = r.dy0 = 1;
-        r.dx1 = r.dy1 = 0;
     }
     int wxs = max(bx0, half);
-    if (wxs <= bx1) {
+    if (wxs <= bx1 && by0 <= by1) {
         r.wx0 = (2 * wxs - (int)W) >> tws;
         r.wx1 = (2 * bx1 - (int)W) >> tws;
         r.wy0 = (by0 + 1) >> ths;
         r.wy1 = (by1 + 1) >> ths;
-    } else {
-        r.wx0 = r.wy0 = 1;
-        r.wx1 = r.wy1 = 0;
     }
 }
 
-// Walks the lane's tiles: the direct rectangle, then the wrapped one minus the
+// Walks a lane's tiles: the direct rectangle, then the wrapped one minus the
 // tiles the direct rectangle already holds.  phase 2 = done.
 struct TileIter {
     int tx, ty, phase;
@@ -101,6 +106,14 @@ __device__ __forceinline__ void iter_settle(const TileRects& r, TileIter& it)
     }
 }
 
+__device__ __forceinline__ void iter_begin(const TileRects& r, TileIter& it)
+{
+    it.phase = 0;
+    it.tx = r.dx0;
+    it.ty = r.dy0;
+    iter_settle(r, it);
+}
+
 __device__ __forceinline__ void iter_next(const TileRects& r, TileIter& it)
 {
     if (it.phase == 0) {
@@ -117,19 +130,62 @@ __device__ __forceinline__ void iter_next(const TileRects& r, TileIter& it)
     iter_settle(r, it);
 }
 
+// One entry appended to tile `gt` at `pos` (when the pool has room).
+template <bool COUNT>
+__device__ __forceinline__ void put_entry(uint64_t gt, uint32_t pos, uint32_t entry, const uint64_t* __restrict__ off,
+                                          uint32_t* __restrict__ pool, uint32_t* __restrict__ overflow, uint32_t frame)
+{
+    if (COUNT) return;
+    const uint64_t o = off[gt], cap = off[gt + 1] - o;
+    if (pos < cap) pool[o + pos] = entry;
+    else overflow[frame] = 1u;
+}
+
+// ---------------------------------------------------------------------------
+// k_bin
+// ---------------------------------------------------------------------------
+constexpr int BIN_WIN = 16;
+
+// Wavefront-wide min / max (all 64 lanes must be active).  A block reduces
+// through these first so that each wavefront issues one LDS atomic: LDS
+// serialises lanes that hit the same address.
+__device__ __forceinline__ int wave_min(int v)
+{
+    for (int sh = 32; sh >= 1; sh >>= 1) v = min(v, __shfl_xor(v, sh));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int sh = 32; sh >= 1; sh >>= 1) v = max(v, __shfl_xor(v, sh));
+    return v;
+}   // the block aggregates appends in a 16 x 16 window of tiles
+
 // Grid: x = blocks of one draw's triangles, y = draw index - d_lo, so the draw,
-// its frame and its matrix are uniform per block (scalar loads).
+// its frame and its matrix are uniform per block (scalar loads).  CLS_BOX
+// triangles are appended to their tiles here; CLS_SPANS / CLS_FULL ones go to
+// the careful list for k_careful.
+//
+// Appends: a block's 256 consecutive triangles lie close together on screen.
+// Lanes whose box touches at most 2 x 2 tiles and does not wrap (nearly all)
+// count themselves into LDS counters of a 16 x 16 tile window anchored at the
+// block's smallest tile coordinates; one global atomicAdd per window tile then
+// reserves that tile's run, and each lane writes its entry at run + its LDS
+// count.  Other lanes append with one global atomicAdd per tile.
 template <bool COUNT>
 __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draws, uint32_t d_lo,
                                                    const Frame* __restrict__ frames, int tws, int ths,
                                                    uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
                                                    uint32_t* __restrict__ pool, int32_t* __restrict__ nl,
-                                                   uint32_t* __restrict__ overflow)
+                                                   uint32_t* __restrict__ overflow, uint32_t* __restrict__ careful_cnt,
+                                                   uint2* __restrict__ careful_in, uint32_t careful_cap)
 {
+    __shared__ uint32_t w_cnt[BIN_WIN * BIN_WIN];
+    __shared__ uint32_t w_base[BIN_WIN * BIN_WIN];
     __shared__ int32_t s_nl[BIN_BLOCK];
-    __shared__ int s_lo, s_hi;
+    __shared__ int s_tx0, s_ty0, s_lo, s_hi;
 
-    const Draw& dr = draws[d_lo + blockIdx.y];
+    const uint32_t d = d_lo + blockIdx.y;
+    const Draw& dr = draws[d];
     const uint32_t base = blockIdx.x * BIN_BLOCK;
     if (base >= dr.n) return;
     const uint32_t local = base + threadIdx.x;
@@ -138,86 +194,196 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     const Frame& fr = frames[frame];
     const uint32_t W = fr.W, H = fr.H;
     const uint32_t tri = dr.tri_begin + local;
-    const int lane = (int)__lane_id();
+    const uint64_t tile_begin = fr.tile_begin;
+    const uint32_t tiles_x = fr.tiles_x;
 
     int y0 = 0, y1d = 0;
-    uint32_t entry = tri;
+    bool careful = false;
     TileRects r;
-    r.dx0 = r.dy0 = r.wx0 = r.wy0 = 1;
-    r.dx1 = r.dy1 = r.wx1 = r.wy1 = 0;
+    empty_rects(r);
     if (valid) {
         Tri t;
         setup_tri(dr.M, dr.xyz + (size_t)local * 9, W, H, t);
         y0 = t.y0;
         y1d = t.y1d;
-        uint32_t cls = setup_class(t);
-        if (cls == CLS_BOX) entry |= BOX_FLAG;
-        int bx0, bx1, by0, by1;
-        if (tri_box(t, cls, bx0, bx1, by0, by1)) tile_rects(bx0, bx1, by0, by1, W, tws, ths, r);
+        const uint32_t cls = setup_class(t);
+        if (cls == CLS_BOX) {
+            const int bx0 = max(t.x0, (int)floorf(t.minx)), bx1 = min(t.x1d - 1, (int)ceilf(t.maxx));
+            if (bx0 <= bx1) tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);
+        } else if (cls != CLS_EMPTY) {
+            careful = true;
+        }
     }
+    const bool has_direct = r.dx0 <= r.dx1 && r.dy0 <= r.dy1;
+    const bool has_wrap = r.wx0 <= r.wx1 && r.wy0 <= r.wy1;
+    const bool small = has_direct && !has_wrap && r.dx1 - r.dx0 <= 1 && r.dy1 - r.dy0 <= 1;
 
+    w_cnt[threadIdx.x] = 0;
+    s_nl[threadIdx.x] = -1;
+    if (threadIdx.x == 0) {
+        s_tx0 = s_ty0 = s_lo = 1 << 30;
+        s_hi = -1;
+    }
+    __syncthreads();
+
+    if (careful) {
+        const uint32_t idx = atomicAdd(careful_cnt, 1u);
+        if (idx < careful_cap) careful_in[idx] = make_uint2(d, local);
+        else if (!COUNT) overflow[frame] = 1u;
+    }
+    const bool has_rows = fr.image && valid && y0 < y1d;
+    {
+        const int wlo = wave_min(has_rows ? y0 : (1 << 30));
+        const int whi = wave_max(has_rows ? y1d - 1 : -1);
+        const int wtx = wave_min(small ? r.dx0 : (1 << 30));
+        const int wty = wave_min(small ? r.dy0 : (1 << 30));
+        if (__lane_id() == 0) {
+            if (whi >= 0) {
+                atomicMin(&s_lo, wlo);
+                atomicMax(&s_hi, whi);
+            }
+            if (wtx < (1 << 30)) {
+                atomicMin(&s_tx0, wtx);
+                atomicMin(&s_ty0, wty);
+            }
+        }
+    }
+    __syncthreads();
+
+    const int ox = s_tx0, oy = s_ty0;
+    const bool inwin = small && r.dx1 - ox < BIN_WIN && r.dy1 - oy < BIN_WIN;
+    const bool wide = r.dx1 > r.dx0, tall = r.dy1 > r.dy0;
+    const int c = (r.dy0 - oy) * BIN_WIN + (r.dx0 - ox);
+    uint32_t lp0 = 0, lp1 = 0, lp2 = 0, lp3 = 0;
+    if (inwin) {
+        lp0 = atomicAdd(&w_cnt[c], 1u);
+        if (wide) lp1 = atomicAdd(&w_cnt[c + 1], 1u);
+        if (tall) lp2 = atomicAdd(&w_cnt[c + BIN_WIN], 1u);
+        if (wide && tall) lp3 = atomicAdd(&w_cnt[c + BIN_WIN + 1], 1u);
+    }
     // rasterizer.rs:89-91: a newline cell after every row of the domain; the
-    // last triangle (in draw order) whose rows include r decides row r.  The
-    // block reduces its rows in LDS and publishes one atomicMax per row.
-    if (fr.image) {
-        if (threadIdx.x == 0) {
-            s_lo = 1 << 30;
-            s_hi = -1;
-        }
-        s_nl[threadIdx.x] = -1;
-        __syncthreads();
-        const bool has_rows = valid && y0 < y1d;
-        if (has_rows) {
-            atomicMin(&s_lo, y0);
-            atomicMax(&s_hi, y1d - 1);
-        }
-        __syncthreads();
-        const int lo = s_lo, hi = s_hi;
-        int32_t* fnl = nl + fr.nl_off;
-        if (hi >= lo) {
-            if (hi - lo < BIN_BLOCK) {
-                if (has_rows)
-                    for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)tri);
-                __syncthreads();
-                const int32_t v = s_nl[threadIdx.x];
-                if ((int)threadIdx.x <= hi - lo && v >= 0) atomicMax(&fnl[lo + (int)threadIdx.x], v);
-            } else if (has_rows) {
-                for (int y = y0; y < y1d; ++y) atomicMax(&fnl[y], (int32_t)tri);
-            }
+    // last triangle (in draw order) whose rows include r decides row r.
+    const int lo = s_lo, hi = s_hi;
+    const bool nl_lds = hi >= lo && hi - lo < BIN_BLOCK;
+    int32_t* fnl = nl + fr.nl_off;
+    if (has_rows) {
+        if (nl_lds)
+            for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)tri);
+        else
+            for (int y = y0; y < y1d; ++y) atomicMax(&fnl[y], (int32_t)tri);
+    }
+    __syncthreads();
+
+    {
+        const uint32_t n = w_cnt[threadIdx.x];
+        if (n) {
+            const uint64_t gt = tile_begin + (uint64_t)(oy + (int)(threadIdx.x / BIN_WIN)) * tiles_x +
+                                (uint64_t)(ox + (int)(threadIdx.x % BIN_WIN));
+            w_base[threadIdx.x] = atomicAdd(&cursor[gt], n);
         }
     }
+    if (nl_lds) {
+        const int32_t v = s_nl[threadIdx.x];
+        if ((int)threadIdx.x <= hi - lo && v >= 0) atomicMax(&fnl[lo + (int)threadIdx.x], v);
+    }
+    __syncthreads();
 
-    // Append the entry to each of its tiles; lanes that share a tile share one
-    // atomicAdd on that tile's cursor (ballot + prefix popcount).
-    const uint64_t tile_begin = fr.tile_begin;
-    const uint32_t tiles_x = fr.tiles_x;
-    TileIter it;
-    it.phase = 0;
-    it.tx = r.dx0;
-    it.ty = r.dy0;
-    iter_settle(r, it);
-    for (;;) {
-        const bool active = it.phase < 2;
-        const uint64_t amask = __ballot(active);
-        if (amask == 0) break;
-        const uint32_t key = active ? (uint32_t)it.ty * tiles_x + (uint32_t)it.tx : 0xFFFFFFFFu;
-        const int leader = __ffsll((unsigned long long)amask) - 1;
-        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
-        const bool match = active && key == lk;
-        const uint64_t mmask = __ballot(match);
-        const uint64_t gt = tile_begin + lk;
-        uint32_t pos0 = 0;
-        if (lane == leader) pos0 = atomicAdd(&cursor[gt], (uint32_t)__popcll(mmask));
-        pos0 = (uint32_t)__builtin_amdgcn_readlane((int)pos0, leader);
-        if (match) {
-            const uint32_t pos = pos0 + (uint32_t)__popcll(mmask & __lanemask_lt());
-            if (!COUNT) {
-                const uint64_t o = off[gt], cap = off[gt + 1] - o;
-                if (pos < cap) pool[o + pos] = entry;
-                else overflow[frame] = 1u;
-            }
+    if (inwin) {
+        const uint64_t g0 = tile_begin + (uint64_t)r.dy0 * tiles_x + (uint64_t)r.dx0;
+        put_entry<COUNT>(g0, w_base[c] + lp0, tri, off, pool, overflow, frame);
+        if (wide) put_entry<COUNT>(g0 + 1, w_base[c + 1] + lp1, tri, off, pool, overflow, frame);
+        if (tall) put_entry<COUNT>(g0 + tiles_x, w_base[c + BIN_WIN] + lp2, tri, off, pool, overflow, frame);
+        if (wide && tall)
+            put_entry<COUNT>(g0 + tiles_x + 1, w_base[c + BIN_WIN + 1] + lp3, tri, off, pool, overflow, frame);
+    } else if (has_direct || has_wrap) {
+        // large or wrapping boxes: one global atomic per tile
+        TileIter it;
+        iter_begin(r, it);
+        while (it.phase < 2) {
+            const uint64_t gt = tile_begin + (uint64_t)it.ty * tiles_x + (uint64_t)it.tx;
+            const uint32_t pos = atomicAdd(&cursor[gt], 1u);
+            put_entry<COUNT>(gt, pos, tri, off, pool, overflow, frame);
             iter_next(r, it);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_careful: exact boxes of the triangles the bound could not decide
+// ---------------------------------------------------------------------------
+// One wavefront per careful triangle.  CLS_SPANS: the lanes split the rows of
+// the domain, each row's exact span comes from row_span(), and the box is the
+// wave's min/max; CLS_FULL: the whole domain.  The lanes then append the
+// triangle to every tile of the box, and lane 0 stores the box for the raster.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws, const Frame* __restrict__ frames,
+                                                 int tws, int ths, uint32_t* __restrict__ cursor,
+                                                 const uint64_t* __restrict__ off, uint32_t* __restrict__ pool,
+                                                 uint32_t* __restrict__ overflow,
+                                                 const uint32_t* __restrict__ careful_cnt,
+                                                 const uint2* __restrict__ careful_in, uint4* __restrict__ careful_box,
+                                                 uint32_t careful_cap)
+{
+    const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = (int)__lane_id();
+    const uint32_t n = min(*careful_cnt, careful_cap);
+    if (w >= n) return;
+    const uint2 in = careful_in[w];
+    const Draw& dr = draws[in.x];
+    const uint32_t frame = dr.frame;
+    const Frame& fr = frames[frame];
+    Tri t;
+    setup_tri(dr.M, dr.xyz + (size_t)in.y * 9, fr.W, fr.H, t);
+    const uint32_t T = dr.tri_begin + in.y;
+    const uint32_t cls = setup_class(t);
+    int bx0 = t.x0, bx1 = t.x1d - 1, by0 = t.y0, by1 = t.y1d - 1;
+    if (cls == CLS_SPANS) {
+        int lo_x = 1 << 30, hi_x = -1, first = 1 << 30, last = -1;
+        for (int y = t.y0 + lane; y < t.y1d; y += 64) {
+            int l, h;
+            if (row_span(t, y, l, h)) {
+                lo_x = min(lo_x, l);
+                hi_x = max(hi_x, h);
+                first = min(first, y);
+                last = max(last, y);
+            }
+        }
+        for (int sh = 32; sh >= 1; sh >>= 1) {
+            lo_x = min(lo_x, __shfl_xor(lo_x, sh));
+            hi_x = max(hi_x, __shfl_xor(hi_x, sh));
+            first = min(first, __shfl_xor(first, sh));
+            last = max(last, __shfl_xor(last, sh));
+        }
+        bx0 = lo_x;
+        bx1 = hi_x;
+        by0 = first;
+        by1 = last;
+    } else if (cls != CLS_FULL) {
+        bx1 = -1;   // decided after all (cannot happen for entries k_bin sent here)
+    }
+    if (lane == 0 && !COUNT)
+        careful_box[w] = make_uint4(T, (uint32_t)max(bx0, 0), (uint32_t)max(bx1, 0),
+                                    (uint32_t)max(by0, 0) | ((uint32_t)max(by1, 0) << 16));
+    if (bx0 > bx1 || by0 > by1) return;
+    TileRects r;
+    tile_rects(bx0, bx1, by0, by1, fr.W, tws, ths, r);
+    const int dw = r.dx1 - r.dx0 + 1, dh = r.dy1 - r.dy0 + 1;
+    const int nd = (r.dx0 <= r.dx1 && r.dy0 <= r.dy1) ? dw * dh : 0;
+    const int ww = r.wx1 - r.wx0 + 1, wh = r.wy1 - r.wy0 + 1;
+    const int nw = (r.wx0 <= r.wx1 && r.wy0 <= r.wy1) ? ww * wh : 0;
+    for (int k = lane; k < nd + nw; k += 64) {
+        int tx, ty;
+        if (k < nd) {
+            tx = r.dx0 + k % dw;
+            ty = r.dy0 + k / dw;
+        } else {
+            tx = r.wx0 + (k - nd) % ww;
+            ty = r.wy0 + (k - nd) / ww;
+            if (nd > 0 && in_direct(r, tx, ty)) continue;
+        }
+        const uint64_t gt = fr.tile_begin + (uint64_t)ty * fr.tiles_x + (uint64_t)tx;
+        const uint32_t pos = atomicAdd(&cursor[gt], 1u);
+        put_entry<COUNT>(gt, pos, CAREFUL_FLAG | w, off, pool, overflow, frame);
     }
 }
 
@@ -227,7 +393,6 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 constexpr uint32_t CACHE_CAP = 256;   // list length up to which the fast path runs
 constexpr uint32_t HASH_CAP = 512;    // open-addressing table: triangle -> cache row
 constexpr uint32_t CACHE_WORDS = 12;
-constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
 
 __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws, const Frame& fr, uint32_t tri)
 {
@@ -266,24 +431,36 @@ __device__ __forceinline__ void walk_tile(const TileGeom& g, int bx0, int bx1, i
         for (int x = xa; x <= xb; ++x) f(x, y, ((y + 1 - g.tr0) << g.sws) + (x - wx0), 0u);
 }
 
-// Candidate box of a pool entry.
-__device__ __forceinline__ bool entry_box(const Tri& t, uint32_t e, int& bx0, int& bx1, int& by0, int& by1)
+// Triangle index of a pool entry and, for careful entries, its box.
+__device__ __forceinline__ uint32_t entry_tri(uint32_t e, const uint4* __restrict__ careful_box, int& bx0, int& bx1,
+                                              int& by0, int& by1, bool& given)
 {
-    if (e & BOX_FLAG) {
-        bx0 = max(t.x0, (int)floorf(t.minx));
-        bx1 = min(t.x1d - 1, (int)ceilf(t.maxx));
-        by0 = t.y0;
-        by1 = t.y1d - 1;
-        return bx0 <= bx1 && by0 <= by1;
-    }
-    return tri_box(t, setup_class(t), bx0, bx1, by0, by1);
+    given = (e & CAREFUL_FLAG) != 0;
+    if (!given) return e;
+    const uint4 b = careful_box[e & ~CAREFUL_FLAG];
+    bx0 = (int)b.y;
+    bx1 = (int)b.z;
+    by0 = (int)(b.w & 0xFFFFu);
+    by1 = (int)(b.w >> 16);
+    return b.x;
+}
+
+// Candidate box of a CLS_BOX entry (the proof in setup_class holds for it).
+__device__ __forceinline__ bool box_of(const Tri& t, int& bx0, int& bx1, int& by0, int& by1)
+{
+    bx0 = max(t.x0, (int)floorf(t.minx));
+    bx1 = min(t.x1d - 1, (int)ceilf(t.maxx));
+    by0 = t.y0;
+    by1 = t.y1d - 1;
+    return bx0 <= bx1 && by0 <= by1;
 }
 
 __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
     const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
     uint32_t f_hi, uint64_t tile_base, int tws, int ths, const uint32_t* __restrict__ cursor,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ pool, const int32_t* __restrict__ nl,
-    uint32_t* __restrict__ cells, float* __restrict__ zbuf, int32_t* __restrict__ win)
+    const uint4* __restrict__ careful_box, uint32_t* __restrict__ cells, float* __restrict__ zbuf,
+    int32_t* __restrict__ win)
 {
     extern __shared__ uint64_t smem[];
     const int sws = tws - 1;
@@ -358,7 +535,12 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
         return g.tr0 + (int)(s >> sws) < g.tr1 && g.tc0 + 2 * (int)(s & ((1u << sws) - 1)) < g.tc1;
     };
 
-    if (n <= CACHE_CAP) {
+    if (n == 0) {
+        // no triangle reaches the tile: blank cells, f32::MAX depths, and the
+        // newline cells of column 1
+        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x)
+            if (slot_valid(s)) store(s, KEY_EMPTY, CELL_BLANK, F32_MAX);
+    } else if (n <= CACHE_CAP) {
         // Fast path: one walk.  Each entry's transformed triangle goes to an
         // LDS row found through a hash of its triangle index; the winner of a
         // slot is re-evaluated from that row.
@@ -369,8 +551,9 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
         for (uint32_t h = threadIdx.x; h < HASH_CAP; h += blockDim.x) htag[h] = EMPTY32;
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
-            const uint32_t e = list[p];
-            const uint32_t T = e & ~BOX_FLAG;
+            int bx0, bx1, by0, by1;
+            bool given;
+            const uint32_t T = entry_tri(list[p], careful_box, bx0, bx1, by0, by1, given);
             const Draw& dr = draw_for(T);
             const uint32_t li = T - dr.tri_begin;
             Tri t;
@@ -390,8 +573,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
                 }
                 h = (h + 1) & (HASH_CAP - 1);
             }
-            int bx0, bx1, by0, by1;
-            if (entry_box(t, e, bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
+            if (given ? (bx0 <= bx1 && by0 <= by1) : box_of(t, bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
         }
         __syncthreads();
         for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
@@ -427,24 +609,24 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
         for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
-            const uint32_t e = list[p];
-            const uint32_t T = e & ~BOX_FLAG;
+            int bx0, bx1, by0, by1;
+            bool given;
+            const uint32_t T = entry_tri(list[p], careful_box, bx0, bx1, by0, by1, given);
             const Draw& dr = draw_for(T);
             Tri t;
             setup_tri(dr.M, dr.xyz + (size_t)(T - dr.tri_begin) * 9, W, H, t);
-            int bx0, bx1, by0, by1;
-            if (entry_box(t, e, bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
+            if (given ? (bx0 <= bx1 && by0 <= by1) : box_of(t, bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
         }
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
-            const uint32_t e = list[p];
-            const uint32_t T = e & ~BOX_FLAG;
+            int bx0, bx1, by0, by1;
+            bool given;
+            const uint32_t T = entry_tri(list[p], careful_box, bx0, bx1, by0, by1, given);
             const Draw& dr = draw_for(T);
             const uint32_t li = T - dr.tri_begin;
             Tri t;
             setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
-            int bx0, bx1, by0, by1;
-            if (!entry_box(t, e, bx0, bx1, by0, by1)) continue;
+            if (!(given ? (bx0 <= bx1 && by0 <= by1) : box_of(t, bx0, bx1, by0, by1))) continue;
             bool have_nz = false;
             float na = 0.0f;
             uint32_t color = 0;
@@ -523,6 +705,7 @@ int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
     if ((rc = d_off.reserve(sizeof(uint64_t) * (n_tiles + 1)))) return rc;
     if ((rc = d_nl.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_rows)))) return rc;
     if ((rc = d_overflow.reserve(sizeof(uint32_t) * std::max<size_t>(1, frames.size())))) return rc;
+    if ((rc = d_careful_cnt.reserve(sizeof(uint32_t)))) return rc;
     return SLOTH_OK;
 }
 
@@ -535,7 +718,8 @@ int TiledPlan::launch_clear(const std::vector<Frame>& frames, uint32_t f_lo, uin
     uint64_t n = std::max(t_hi - t_lo, r_hi - r_lo);
     unsigned g = std::max(1u, std::min(blocks(n, 256), 4096u));
     hipLaunchKernelGGL(k_tile_clear, dim3(g), dim3(256), 0, s, d_cursor.as<uint32_t>() + t_lo, t_hi - t_lo,
-                       d_nl.as<int32_t>() + r_lo, r_hi - r_lo, d_overflow.as<uint32_t>() + f_lo, f_hi - f_lo);
+                       d_nl.as<int32_t>() + r_lo, r_hi - r_lo, d_overflow.as<uint32_t>() + f_lo, f_hi - f_lo,
+                       d_careful_cnt.as<uint32_t>());
     HIP_TRY(hipGetLastError());
     return SLOTH_OK;
 }
@@ -554,13 +738,34 @@ int TiledPlan::launch_bin(const std::vector<Frame>& frames, const std::vector<Dr
         if (count_only)
             hipLaunchKernelGGL(k_bin<true>, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
                                d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
-                               d_nl.as<int32_t>(), d_overflow.as<uint32_t>());
+                               d_nl.as<int32_t>(), d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(),
+                               d_careful_in.as<uint2>(), careful_cap);
         else
             hipLaunchKernelGGL(k_bin<false>, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
                                d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
-                               d_nl.as<int32_t>(), d_overflow.as<uint32_t>());
+                               d_nl.as<int32_t>(), d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(),
+                               d_careful_in.as<uint2>(), careful_cap);
         HIP_TRY(hipGetLastError());
     }
+    return SLOTH_OK;
+}
+
+int TiledPlan::launch_careful(const Frame* d_frames, const Draw* d_draws, bool count_only, hipStream_t s)
+{
+    if (careful_cap == 0) return SLOTH_OK;
+    const int tws = log2u(TW), ths = log2u(TH);
+    dim3 grid(blocks(careful_cap, 4));   // 4 wavefronts per 256-thread block, one per careful triangle
+    if (count_only)
+        hipLaunchKernelGGL(k_careful<true>, grid, dim3(256), 0, s, d_draws, d_frames, tws, ths,
+                           d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
+                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), d_careful_in.as<uint2>(),
+                           d_careful_box.as<uint4>(), careful_cap);
+    else
+        hipLaunchKernelGGL(k_careful<false>, grid, dim3(256), 0, s, d_draws, d_frames, tws, ths,
+                           d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
+                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), d_careful_in.as<uint2>(),
+                           d_careful_box.as<uint4>(), careful_cap);
+    HIP_TRY(hipGetLastError());
     return SLOTH_OK;
 }
 
@@ -570,8 +775,22 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     if (frames.empty()) return SLOTH_OK;
     const uint32_t F = (uint32_t)frames.size();
     int rc;
+    // 1. count tile entries of CLS_BOX triangles and the careful triangles
     if ((rc = launch_clear(frames, 0, F, s))) return rc;
     if ((rc = launch_bin(frames, draws, d_frames, d_draws, 0, F, true, s))) return rc;
+    uint32_t n_careful = 0;
+    HIP_TRY(hipMemcpyAsync(&n_careful, d_careful_cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (n_careful > careful_cap) {
+        // 2. room for the careful list, then count again so that it is filled
+        careful_cap = n_careful + n_careful / 4 + 256;
+        if ((rc = d_careful_in.reserve(sizeof(uint2) * careful_cap))) return rc;
+        if ((rc = d_careful_box.reserve(sizeof(uint4) * careful_cap))) return rc;
+        if ((rc = launch_clear(frames, 0, F, s))) return rc;
+        if ((rc = launch_bin(frames, draws, d_frames, d_draws, 0, F, true, s))) return rc;
+    }
+    // 3. the careful triangles' tile entries
+    if ((rc = launch_careful(d_frames, d_draws, true, s))) return rc;
     std::vector<uint32_t> counts(n_tiles);
     if (n_tiles)
         HIP_TRY(hipMemcpyAsync(counts.data(), d_cursor.p, sizeof(uint32_t) * n_tiles, hipMemcpyDeviceToHost, s));
@@ -601,16 +820,19 @@ int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>&
     if (ev) HIP_TRY(hipEventRecord((*ev)[1], s));
     if ((rc = launch_bin(frames, draws, d_frames, d_draws, f_lo, f_hi, false, s))) return rc;
     if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
+    if ((rc = launch_careful(d_frames, d_draws, false, s))) return rc;
+    if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
     const Frame& a = frames[f_lo];
     const Frame& b = frames[f_hi - 1];
     uint64_t t_lo = a.tile_begin, t_hi = b.tile_begin + (uint64_t)b.tiles_x * b.tiles_y;
     if (t_hi > t_lo) {
         hipLaunchKernelGGL(k_tile_raster, dim3((unsigned)(t_hi - t_lo)), dim3(RASTER_BLOCK), raster_lds(), s, d_draws,
                            d_ramps, d_frames, f_lo, f_hi, t_lo, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
-                           d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(), cells, z, win);
+                           d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_careful_box.as<uint4>(),
+                           cells, z, win);
         HIP_TRY(hipGetLastError());
     }
-    if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
+    if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));
     return SLOTH_OK;
 }
 
