@@ -50,10 +50,12 @@ def main() -> int:
     ap.add_argument("--profile-runs", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--graph-only", action="store_true",
-                    help="only the batched graph renders (for profilers): no plain, stage or CPU legs")
+                    help="only the batched graph renders (for profilers): no plain, stage, CPU or other legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 4K, uneven-batch and shading legs")
     args = ap.parse_args()
     if args.graph_only:
         args.plain_steps = args.profile_runs = args.cpu_frames = 0
+        args.no_extra = True
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -139,6 +141,12 @@ def main() -> int:
         fr.draw_mesh(U, m.xyz, m.rgb, Rs[0])
         frags = int(fr.stats[1])
 
+    extra = {}
+    if not args.no_extra:
+        extra["frame_4k"] = leg_4k(sloth_amd, meshgen)
+        extra["uneven_batch"] = leg_uneven(sloth_amd, meshgen)
+        extra["fwd_bwd"] = leg_shading(sloth_amd, meshgen)
+
     out = {
         "metric": METRIC,
         "value": round(fps, 2),
@@ -163,11 +171,81 @@ def main() -> int:
         "fragments_per_frame": frags,
         "atomics_per_s": round(frags * fps, 1) if frags else None,
         "verified_frames": verified,
+        "fwd_bwd_views_per_s": extra.get("fwd_bwd", {}).get("views_per_s"),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "legs": extra,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def leg_4k(sloth_amd, meshgen):
+    """One 3840x2160 frame of the 8-subdivision displaced icosphere
+    (1,310,720 triangles, seed 1), plain form (launch sequence + sync)."""
+    log("bench: 4K frame leg")
+    m, W, H, U, R = meshgen.workload_icosphere_4k()
+    mesh = sloth_amd.SimpleMesh.from_mesh(m)
+    b = sloth_amd.Batch([sloth_amd.FrameSpec(W, H, U, True)], [sloth_amd.DrawSpec(0, mesh, R)])
+    ms = b.time(10, 2, per_frame=True) / 10
+    return {"workload": f"displaced icosphere s8 ({m.n} tris, seed 1), 3840x2160 image mode, one frame",
+            "ms_per_frame": round(ms, 4), "frames_per_s": round(1e3 / ms, 1),
+            "gtri_per_s": round(m.n / ms / 1e6, 3)}
+
+
+def leg_uneven(sloth_amd, meshgen):
+    """64 meshes, triangle counts log-uniform in 1k..1M (seed 3), each with its
+    own resolution and rotation, rendered by one batch launch sequence."""
+    log("bench: uneven-batch leg (generating 64 meshes)")
+    items = meshgen.workload_uneven_meshes()
+    frames, draws, tris = [], [], 0
+    keep = []
+    for i, (m, W, H, U, R) in enumerate(items):
+        sm = sloth_amd.SimpleMesh.from_mesh(m)
+        keep.append(sm)
+        frames.append(sloth_amd.FrameSpec(W, H, U, True))
+        draws.append(sloth_amd.DrawSpec(i, sm, R))
+        tris += m.n
+    b = sloth_amd.Batch(frames, draws)
+    ms = b.time(5, 2) / 5
+    pms = b.time(1, 1, per_frame=True)
+    return {"workload": f"64 meshes, {tris} triangles total (1k..1M log-uniform, seed 3), resolutions from "
+                        f"{sorted(set((f.width, f.height) for f in frames))}",
+            "ms_per_batch": round(ms, 4), "meshes_per_s": round(64e3 / ms, 1),
+            "gtri_per_s": round(tris / ms / 1e6, 3), "plain_ms": round(pms, 4),
+            "batched_over_plain": round(pms / ms, 2)}
+
+
+def leg_shading(sloth_amd, meshgen):
+    """Differentiable shading: UV sphere 224 x 224 (99,904 triangles), 32
+    seeded views at 1024x1024, L2 loss against a target rendered with colours
+    + N(0, 0.1^2) and the light rotated 10 degrees; one step = forward +
+    backward over all 32 views with visibility fixed."""
+    import math
+
+    log("bench: shading fwd+bwd leg")
+    m = meshgen.uv_sphere(224, 224, 1.0, seed=5)
+    rng = np.random.default_rng(5)
+    W = H = 1024
+    V = 32
+    U = meshgen.screen_transform(W, H, [m.bbox_max()])
+    mesh = sloth_amd.SimpleMesh.from_mesh(m)
+    Rs = [meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3)) for _ in range(V)]
+    b = sloth_amd.Batch([sloth_amd.FrameSpec(W, H, U, True) for _ in range(V)],
+                        [sloth_amd.DrawSpec(f, mesh, Rs[f]) for f in range(V)])
+    sh = sloth_amd.Shading(b, m.faces, m.verts.shape[0])
+    vc = rng.uniform(0, 1, (m.verts.shape[0], 3)).astype(np.float32)
+    vn = (m.verts / np.linalg.norm(m.verts, axis=1, keepdims=True)).astype(np.float32)
+    L = np.array([0.577, 0.577, -0.577], np.float32)
+    a = math.radians(10.0)
+    Ry = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+    _, target = sh.forward((vc + rng.normal(0, 0.1, vc.shape)).astype(np.float32), vn,
+                           (Ry @ L.astype(np.float64)).astype(np.float32), want_out=True)
+    sh.set_target(target)
+    loss, _ = sh.forward(vc, vn, L)
+    ms = sh.time(10, 2) / 10
+    return {"workload": f"UV sphere 224x224 ({m.n} tris), {V} views 1024x1024, fwd+bwd, visibility fixed",
+            "ms_per_step": round(ms, 4), "views_per_s": round(V * 1e3 / ms, 1), "loss": loss}
 
 
 if __name__ == "__main__":

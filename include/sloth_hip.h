@@ -193,6 +193,45 @@ int sloth_batch_time(sloth_batch *b, uint32_t steps, uint32_t warmup, int mode, 
 
 int sloth_batch_destroy(sloth_batch *b);
 
+/* ---- differentiable shading (no reference counterpart: the reference shades
+ * with a flat glyph, rasterizer.rs:76; this is the build's added form) ---- */
+
+/* Shades the visible fragments of a batch with per-vertex colours, per-vertex
+ * normals and a light direction, with visibility fixed by the raster.  For
+ * each covered z slot (winning triangle t, pixel p) with barycentrics
+ * lambda_k = w_k(p) / orient(v1,v2,v3) (rasterizer.rs:67-74):
+ *     rgb = (L . sum_k lambda_k G n[f_k]) * (sum_k lambda_k c[f_k])
+ * where f = faces[t] and G = cof(M3) / |det M3|^(2/3) maps model-space normals
+ * to the screen space of Triangle::normal (M3 = upper-left 3x3 of
+ * utransform * transform).  With n = the face normal and L = (0,0,1) the
+ * intensity is the reference's flat shade.  The loss is the sum over covered
+ * slots and channels of (rgb - target)^2.
+ *
+ * faces: [n_tris][3] vertex ids shared by every draw of the batch (every draw
+ * must draw a mesh of n_tris triangles).  Creating the shading renders the
+ * batch once to fix the visibility. */
+typedef struct sloth_shading sloth_shading;
+int sloth_shading_create(sloth_batch *b, const uint32_t *faces, uint32_t n_tris, uint32_t n_verts,
+                         sloth_shading **out);
+/* Re-renders the visibility (after sloth_batch_set_transforms). */
+int sloth_shading_visibility(sloth_shading *s);
+/* Number of z slots over all frames (the rows of target and out_rgb). */
+int sloth_shading_slots(const sloth_shading *s, uint64_t *n_slots);
+/* target: float[n_slots][3] on the host (copied); NULL removes it. */
+int sloth_shading_set_target(sloth_shading *s, const float *target);
+/* Forward: vcolor, vnormal float[n_verts][3], light float[3].  out_rgb
+ * (float[n_slots][3], zero where no triangle is visible) and loss (needs a
+ * target) may be NULL. */
+int sloth_shading_forward(sloth_shading *s, const float *vcolor, const float *vnormal, const float *light,
+                          float *out_rgb, double *loss);
+/* Backward of the loss at the parameters of the last forward call: gradients
+ * float[n_verts][3], float[n_verts][3], float[3] (any may be NULL). */
+int sloth_shading_backward(sloth_shading *s, float *d_vcolor, float *d_vnormal, float *d_light);
+/* Times `steps` forward+backward passes on the device-resident parameters
+ * and target after `warmup` untimed ones; *ms = elapsed milliseconds. */
+int sloth_shading_time(sloth_shading *s, uint32_t steps, uint32_t warmup, float *ms);
+int sloth_shading_destroy(sloth_shading *s);
+
 /* ---- helpers ---------------------------------------------------------- */
 
 /* Context::update's utransform for a size and bounding-box maxima

@@ -56,8 +56,12 @@ def lib():
         L.oracle_ramp_shade.argtypes = [C.POINTER(Ramp), C.c_float]
         L.oracle_ramp_shade.restype = C.c_uint8
         L.oracle_default_ramp.argtypes = [C.POINTER(Ramp)]
-        L.oracle_draw_mesh.argtypes = [C.c_uint32, C.c_uint32, C.c_int, fp, u32p, fp, i32p, fp, u8p,
+        L.oracle_draw_mesh.argtypes = [C.c_uint32, C.c_uint32, C.c_int, fp, u32p, fp, i32p, i32p, fp, u8p,
                                        C.c_uint32, fp, C.POINTER(Ramp), C.c_uint32, u64p]
+        dp = C.POINTER(C.c_double)
+        L.oracle_shade_f64.argtypes = [C.c_uint32, C.c_uint32, fp, fp, i32p, C.c_uint32, dp, dp, dp, i32p, i32p,
+                                       fp, dp, dp, dp, dp]
+        L.oracle_shade_f64.restype = C.c_double
         L.oracle_update.argtypes = [C.c_uint32, C.c_uint32, fp, C.c_uint32, fp]
         L.oracle_mesh_bbox.argtypes = [fp, C.c_uint32, fp]
         L.oracle_clear.argtypes = [C.c_uint32, C.c_uint32, C.c_int, u32p, fp]
@@ -153,6 +157,7 @@ class Frame:
         self.cells = np.empty(width * height + (height if image else 0), np.uint32)
         self.z = np.empty(width * height, np.float32)
         self.win = np.full(width * height, -1, np.int32)
+        self.winxy = np.full(2 * width * height, -1, np.int32)   # winning pixel (x, y) per z slot
         self.tri_base = 0
         self.stats = np.zeros(3, np.uint64)   # candidates, inside, z-passes
         self.clear()
@@ -161,6 +166,7 @@ class Frame:
         lib().oracle_clear(self.W, self.H, int(self.image), _p(self.cells, C.c_uint32),
                            _p(self.z, C.c_float))
         self.win[:] = -1
+        self.winxy[:] = -1
         self.tri_base = 0
 
     def draw_mesh(self, U, xyz, rgb, R, ramp: Ramp | None = None):
@@ -172,7 +178,8 @@ class Frame:
         n = xyz.shape[0]
         lib().oracle_draw_mesh(self.W, self.H, int(self.image), _p(U, C.c_float),
                                _p(self.cells, C.c_uint32), _p(self.z, C.c_float),
-                               _p(self.win, C.c_int32), _p(xyz, C.c_float), _p(rgb, C.c_uint8),
+                               _p(self.win, C.c_int32), _p(self.winxy, C.c_int32), _p(xyz, C.c_float),
+                               _p(rgb, C.c_uint8),
                                n, _p(R, C.c_float), C.byref(ramp), self.tri_base,
                                _p(self.stats, C.c_uint64))
         self.tri_base += n
@@ -216,3 +223,28 @@ def render_frames(width, height, U, Rs, xyz, rgb, image=True, nthreads=1, keep=T
 
 def shader_exhaustive_mismatch(lo: int = 0, hi: int = 65536) -> int:
     return int(lib().oracle_shader_exhaustive_mismatch(lo, hi))
+
+
+def shade_f64(frame: Frame, M, xyz, faces, vcolor, vnormal, light, target=None, want_out=False,
+              want_grad=False):
+    """Differentiable-shading model in float64 over the visible fragments of
+    `frame` (one draw; M = utransform * transform).  Returns
+    (loss, out [W*H,3] or None, (dcolor, dnormal, dlight) or None)."""
+    M = np.ascontiguousarray(M, np.float32).reshape(16)
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    faces = np.ascontiguousarray(faces, np.int32)
+    vc = np.ascontiguousarray(vcolor, np.float64)
+    vn = np.ascontiguousarray(vnormal, np.float64)
+    L = np.ascontiguousarray(light, np.float64)
+    nv = vc.shape[0]
+    dp = C.POINTER(C.c_double)
+    tgt = None if target is None else np.ascontiguousarray(target, np.float32)
+    out = np.zeros((frame.W * frame.H, 3)) if want_out else None
+    grads = (np.zeros((nv, 3)), np.zeros((nv, 3)), np.zeros(3)) if want_grad else None
+    loss = lib().oracle_shade_f64(
+        frame.W, frame.H, _p(M, C.c_float), _p(xyz, C.c_float), _p(faces, C.c_int32), nv, _p(vc, C.c_double),
+        _p(vn, C.c_double), _p(L, C.c_double), _p(frame.win, C.c_int32), _p(frame.winxy, C.c_int32),
+        _p(tgt, C.c_float) if tgt is not None else None, _p(out, C.c_double) if out is not None else None,
+        _p(grads[0], C.c_double) if grads else None, _p(grads[1], C.c_double) if grads else None,
+        _p(grads[2], C.c_double) if grads else None)
+    return float(loss), out, grads

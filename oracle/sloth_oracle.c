@@ -202,7 +202,7 @@ static inline uint64_t as_usize(float f)
  * triangle, the reference merely recomputes it).  Updates z, cells and, if
  * given, win (winning global triangle index per z slot, -1 = none). */
 static void draw_triangle(uint32_t W, uint32_t H, int image, const float *UR,
-                          float *zb, uint32_t *cells, int32_t *win, const float *src,
+                          float *zb, uint32_t *cells, int32_t *win, int32_t *winxy, const float *src,
                           uint8_t r, uint8_t g, uint8_t b, const oracle_ramp *ramp,
                           int32_t tri_index, uint64_t *stats)
 {
@@ -238,6 +238,10 @@ static void draw_triangle(uint32_t W, uint32_t H, int image, const float *UR,
                     cells[id] = cell;
                     cells[id + 1] = cell;
                     if (win) win[id] = tri_index;
+                    if (winxy) {
+                        winxy[2 * id] = (int32_t)x;
+                        winxy[2 * id + 1] = (int32_t)y;
+                    }
                     if (stats) stats[2]++;
                 }
             }
@@ -251,8 +255,9 @@ static void draw_triangle(uint32_t W, uint32_t H, int image, const float *UR,
  * holds [n][3] bytes.  tri_base is the number of triangles drawn earlier in
  * the same frame (global draw order = z-tie order). */
 void oracle_draw_mesh(uint32_t W, uint32_t H, int image, const float *U, uint32_t *cells,
-                      float *zb, int32_t *win, const float *xyz, const uint8_t *rgb, uint32_t n,
-                      const float *R, const oracle_ramp *ramp, uint32_t tri_base, uint64_t *stats)
+                      float *zb, int32_t *win, int32_t *winxy, const float *xyz, const uint8_t *rgb,
+                      uint32_t n, const float *R, const oracle_ramp *ramp, uint32_t tri_base,
+                      uint64_t *stats)
 {
     float UR[16];
     for (uint32_t i = 0; i < n; ++i) {
@@ -265,7 +270,7 @@ void oracle_draw_mesh(uint32_t W, uint32_t H, int image, const float *U, uint32_
             tri[4 * k + 2] = xyz[(size_t)i * 9 + 3 * k + 2];
             tri[4 * k + 3] = 1.0f;
         }
-        draw_triangle(W, H, image, UR, zb, cells, win, tri, rgb[(size_t)i * 3 + 0],
+        draw_triangle(W, H, image, UR, zb, cells, win, winxy, tri, rgb[(size_t)i * 3 + 0],
                       rgb[(size_t)i * 3 + 1], rgb[(size_t)i * 3 + 2], ramp,
                       (int32_t)(tri_base + i), stats);
     }
@@ -370,8 +375,8 @@ static void *frames_worker(void *arg)
             break;
         }
         oracle_clear(j->W, j->H, j->image, cells, zb);
-        oracle_draw_mesh(j->W, j->H, j->image, j->U + 16 * (size_t)f, cells, zb, NULL, j->xyz,
-                         j->rgb, j->n, j->R + 16 * (size_t)f, j->ramp, 0, NULL);
+        oracle_draw_mesh(j->W, j->H, j->image, j->U + 16 * (size_t)f, cells, zb, NULL, NULL,
+                         j->xyz, j->rgb, j->n, j->R + 16 * (size_t)f, j->ramp, 0, NULL);
         if (j->cells_out) memcpy(j->cells_out + nc * f, cells, nc * sizeof(uint32_t));
         if (j->z_out) memcpy(j->z_out + nz * f, zb, nz * sizeof(float));
         free(cells);
@@ -417,4 +422,109 @@ uint32_t oracle_shader_exhaustive_mismatch(uint32_t lo_hi16, uint32_t hi_hi16)
         }
     }
     return bad;
+}
+
+/* ------------------------------------------------------------------ */
+/* Differentiable shading, FP64 (no reference counterpart)              */
+/* ------------------------------------------------------------------ */
+
+/* The shading model of the build's added differentiable form.  Visibility
+ * is fixed: win[id] / winxy[id] give the triangle and the pixel that own z
+ * slot id (from oracle_draw_mesh).  For that pixel, with the screen triangle
+ * of draw_triangle (binary32 transform, rasterizer.rs:56-57):
+ *   lambda_k = w_k / orient(v1,v2,v3)        (w_k as rasterizer.rs:72-74)
+ *   G        = cof(M3) / |det M3|^(2/3)      (maps model-space normals to
+ *              the screen-space normal of Triangle::normal, scale-free)
+ *   I        = L . sum_k lambda_k G n[v_k]
+ *   C        = sum_k lambda_k c[v_k]
+ *   out      = I * C                         (RGB)
+ *   loss     = sum over covered slots, channels of (out - target)^2
+ * With n[v_k] the face's unit normal and L = (0,0,1), I is the reference's
+ * flat shade nz_unit * a * (w0 + w1 + w2).  Everything below is float64.
+ * Gradients (if non-NULL) are added into dcolor [nv][3], dnormal [nv][3],
+ * dlight [3].  Returns the loss (0 without a target). */
+double oracle_shade_f64(uint32_t W, uint32_t H, const float *M, const float *xyz, const int32_t *faces,
+                        uint32_t nv, const double *vcolor, const double *vnormal, const double *light,
+                        const int32_t *win, const int32_t *winxy, const float *target, double *out,
+                        double *dcolor, double *dnormal, double *dlight)
+{
+    (void)nv;
+    /* G = cof(M3) / |det M3|^(2/3), M3 = upper-left 3x3 of M (column-major) */
+    double m[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m[r][c] = (double)M[c * 4 + r];
+    double cof[3][3];
+    cof[0][0] = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    cof[0][1] = -(m[1][0] * m[2][2] - m[1][2] * m[2][0]);
+    cof[0][2] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    cof[1][0] = -(m[0][1] * m[2][2] - m[0][2] * m[2][1]);
+    cof[1][1] = m[0][0] * m[2][2] - m[0][2] * m[2][0];
+    cof[1][2] = -(m[0][0] * m[2][1] - m[0][1] * m[2][0]);
+    cof[2][0] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+    cof[2][1] = -(m[0][0] * m[1][2] - m[0][2] * m[1][0]);
+    cof[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+    double det = m[0][0] * cof[0][0] + m[0][1] * cof[0][1] + m[0][2] * cof[0][2];
+    double k = pow(fabs(det), 2.0 / 3.0);
+    double G[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) G[r][c] = cof[r][c] / k;
+
+    double loss = 0.0;
+    uint64_t nz = (uint64_t)W * H;
+    for (uint64_t id = 0; id < nz; ++id) {
+        int32_t T = win[id];
+        if (T < 0) continue;
+        double px = winxy[2 * id], py = winxy[2 * id + 1];
+        float t[12];
+        for (int v = 0; v < 3; ++v) {
+            t[4 * v + 0] = xyz[(size_t)T * 9 + 3 * v + 0];
+            t[4 * v + 1] = xyz[(size_t)T * 9 + 3 * v + 1];
+            t[4 * v + 2] = xyz[(size_t)T * 9 + 3 * v + 2];
+            t[4 * v + 3] = 1.0f;
+        }
+        oracle_tri_mul(M, t);
+        double x1 = t[0], y1 = t[1], x2 = t[4], y2 = t[5], x3 = t[8], y3 = t[9];
+        double w[3];
+        w[0] = (x3 - x2) * (py - y2) - (y3 - y2) * (px - x2);
+        w[1] = (x1 - x3) * (py - y3) - (y1 - y3) * (px - x3);
+        w[2] = (x2 - x1) * (py - y1) - (y2 - y1) * (px - x1);
+        double o = (x2 - x1) * (y3 - y1) - (y2 - y1) * (x3 - x1);
+        double lam[3] = {w[0] / o, w[1] / o, w[2] / o};
+        double nt[3][3], n[3] = {0, 0, 0}, C[3] = {0, 0, 0};
+        for (int v = 0; v < 3; ++v) {
+            int32_t vi = faces[(size_t)T * 3 + v];
+            for (int r = 0; r < 3; ++r)
+                nt[v][r] = G[r][0] * vnormal[3 * vi + 0] + G[r][1] * vnormal[3 * vi + 1] + G[r][2] * vnormal[3 * vi + 2];
+            for (int r = 0; r < 3; ++r) {
+                n[r] += lam[v] * nt[v][r];
+                C[r] += lam[v] * vcolor[3 * vi + r];
+            }
+        }
+        double I = light[0] * n[0] + light[1] * n[1] + light[2] * n[2];
+        double O[3] = {I * C[0], I * C[1], I * C[2]};
+        if (out)
+            for (int r = 0; r < 3; ++r) out[3 * id + r] = O[r];
+        if (!target) continue;
+        double g[3];
+        for (int r = 0; r < 3; ++r) {
+            double d = O[r] - (double)target[3 * id + r];
+            loss += d * d;
+            g[r] = 2.0 * d;
+        }
+        if (!dcolor && !dnormal && !dlight) continue;
+        double dI = g[0] * C[0] + g[1] * C[1] + g[2] * C[2];
+        for (int v = 0; v < 3; ++v) {
+            int32_t vi = faces[(size_t)T * 3 + v];
+            if (dcolor)
+                for (int r = 0; r < 3; ++r) dcolor[3 * vi + r] += lam[v] * g[r] * I;
+            if (dnormal) {
+                /* d/dn[v] of L . (lam G n[v]) = lam G^T L */
+                for (int c = 0; c < 3; ++c)
+                    dnormal[3 * vi + c] += dI * lam[v] * (G[0][c] * light[0] + G[1][c] * light[1] + G[2][c] * light[2]);
+            }
+        }
+        if (dlight)
+            for (int r = 0; r < 3; ++r) dlight[r] += dI * n[r];
+    }
+    return loss;
 }

@@ -27,8 +27,13 @@ uint8_t oracle_default_shader(float shade);
 uint8_t oracle_ramp_shade(const oracle_ramp *r, float shade);
 void oracle_default_ramp(oracle_ramp *r);
 void oracle_draw_mesh(uint32_t W, uint32_t H, int image, const float *U, uint32_t *cells,
-                      float *zb, int32_t *win, const float *xyz, const uint8_t *rgb, uint32_t n,
-                      const float *R, const oracle_ramp *ramp, uint32_t tri_base, uint64_t *stats);
+                      float *zb, int32_t *win, int32_t *winxy, const float *xyz, const uint8_t *rgb,
+                      uint32_t n, const float *R, const oracle_ramp *ramp, uint32_t tri_base,
+                      uint64_t *stats);
+double oracle_shade_f64(uint32_t W, uint32_t H, const float *M, const float *xyz, const int32_t *faces,
+                        uint32_t nv, const double *vcolor, const double *vnormal, const double *light,
+                        const int32_t *win, const int32_t *winxy, const float *target, double *out,
+                        double *dcolor, double *dnormal, double *dlight);
 void oracle_update(uint32_t W, uint32_t H, const float *bbox_max, uint32_t n_meshes, float *U);
 void oracle_mesh_bbox(const float *xyz, uint32_t n, float *out);
 void oracle_clear(uint32_t W, uint32_t H, int image, uint32_t *cells, float *zb);

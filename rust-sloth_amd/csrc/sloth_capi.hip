@@ -18,6 +18,7 @@
 #include "sloth_hip.h"
 #include "sloth_host.h"
 #include "sloth_kernels.h"
+#include "sloth_shade.h"
 #include "sloth_tiled.h"
 
 using namespace sloth;
@@ -219,8 +220,8 @@ struct Job {
                 d.ramp = di.ramp;
                 d.item_begin = items;
                 memcpy(d.M, di.M, sizeof d.M);
-                if ((uint64_t)tri + d.n >= (1ull << 31))
-                    return fail(SLOTH_EINVAL, "frame %u has 2^31 or more triangles", f);
+                if ((uint64_t)tri + d.n >= (1ull << 31) - 1)
+                    return fail(SLOTH_EINVAL, "frame %u has 2^31 - 1 or more triangles", f);
                 tri += d.n;
                 items += d.n;
                 ++k;
@@ -285,7 +286,8 @@ struct Job {
     }
 
     // Global path over frames [f_lo, f_hi).
-    int launch_global(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, std::vector<hipEvent_t>* ev)
+    int launch_global(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, bool win_raw,
+                      std::vector<hipEvent_t>* ev)
     {
         const Frame& a = frames[f_lo];
         const Frame& b = frames[f_hi - 1];
@@ -315,7 +317,8 @@ struct Job {
         if (c_hi > c_lo) {
             hipLaunchKernelGGL(k_resolve_global, dim3(blocks_for(c_hi - c_lo)), dim3(256), 0, s, d_draws.as<Draw>(),
                                d_ramps.as<Ramp>(), d_frames.as<Frame>(), f_lo, f_hi, c_lo, c_hi - c_lo,
-                               d_keys.as<uint64_t>(), d_nl.as<int32_t>(), d_cells.as<uint32_t>(), d_z.as<float>(), win);
+                               d_keys.as<uint64_t>(), d_nl.as<int32_t>(), d_cells.as<uint32_t>(), d_z.as<float>(), win,
+                               win_raw ? 1u : 0u);
             HIP_TRY(hipGetLastError());
         }
         if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
@@ -334,7 +337,8 @@ struct Job {
     // After a render of frames [f_lo, f_hi) completed on `s`: if a tile list
     // overflowed, re-size the lists and render those frames again.
     // *replanned tells the caller that device pointers changed.
-    int finish(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, bool* replanned = nullptr)
+    int finish(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, bool* replanned = nullptr,
+               bool win_raw = false)
     {
         if (replanned) *replanned = false;
         if (path != PATH_TILED) return SLOTH_OK;
@@ -346,18 +350,19 @@ struct Job {
             tiled.sized = false;
             if (replanned) *replanned = true;
             if ((rc = prepare(s))) return rc;
-            if ((rc = launch(f_lo, f_hi, s, win))) return rc;
+            if ((rc = launch(f_lo, f_hi, s, win, nullptr, win_raw))) return rc;
         }
         return fail(SLOTH_EHIP, "tile lists kept overflowing");
     }
 
-    int launch(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, std::vector<hipEvent_t>* ev = nullptr)
+    int launch(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, std::vector<hipEvent_t>* ev = nullptr,
+               bool win_raw = false)
     {
         if (f_lo >= f_hi) return SLOTH_OK;
         if (path == PATH_TILED)
             return tiled.launch(frames, draws, d_frames.as<Frame>(), d_draws.as<Draw>(), d_ramps.as<Ramp>(), f_lo,
-                                f_hi, d_cells.as<uint32_t>(), d_z.as<float>(), win, s, ev);
-        return launch_global(f_lo, f_hi, s, win, ev);
+                                f_hi, d_cells.as<uint32_t>(), d_z.as<float>(), win, win_raw, s, ev);
+        return launch_global(f_lo, f_hi, s, win, win_raw, ev);
     }
 
     int ensure_win()
@@ -961,6 +966,285 @@ int sloth_batch_destroy(sloth_batch* b)
 {
     if (b && b->stream) (void)hipStreamSynchronize(b->stream);
     delete b;
+    return SLOTH_OK;
+}
+
+
+// ---- differentiable shading -------------------------------------------------
+
+}  // extern "C"
+
+struct sloth_shading {
+    sloth_batch* b = nullptr;
+    uint32_t n_tris = 0, n_verts = 0;
+    uint64_t n_slots = 0;
+    bool has_target = false;
+    bool has_params = false;
+    DevBuf d_faces, d_vis, d_G, d_vcolor, d_vnormal, d_light, d_target, d_out, d_partial, d_tri_acc;
+    DevBuf d_adj_off, d_adj, d_dcolor, d_dnormal;
+    std::vector<double> partial;
+};
+
+static ShadeArgs shade_args(sloth_shading* sh, bool with_target, float* out)
+{
+    ShadeArgs a;
+    Job& j = sh->b->job;
+    a.frames = j.d_frames.as<Frame>();
+    a.n_frames = (uint32_t)j.frames.size();
+    a.draws = j.d_draws.as<Draw>();
+    a.vis = sh->d_vis.as<int32_t>();
+    a.faces = sh->d_faces.as<uint32_t>();
+    a.vcolor = sh->d_vcolor.as<float>();
+    a.vnormal = sh->d_vnormal.as<float>();
+    a.light = sh->d_light.as<float>();
+    a.G = sh->d_G.as<float>();
+    a.target = with_target ? sh->d_target.as<float>() : nullptr;
+    a.out = out;
+    a.partial = sh->d_partial.as<double>();
+    a.tri_acc = sh->d_tri_acc.as<double>();
+    a.n_slots = sh->n_slots;
+    a.n_tris = sh->n_tris;
+    a.n_verts = sh->n_verts;
+    return a;
+}
+
+// Normal map of a draw: cof(M3) / |det M3|^(2/3) (M3 = upper-left 3x3 of
+// utransform * transform), row-major, computed in double.
+static void normal_map(const float* M, float* G)
+{
+    double m[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m[r][c] = (double)M[c * 4 + r];
+    double cof[3][3];
+    cof[0][0] = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    cof[0][1] = -(m[1][0] * m[2][2] - m[1][2] * m[2][0]);
+    cof[0][2] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    cof[1][0] = -(m[0][1] * m[2][2] - m[0][2] * m[2][1]);
+    cof[1][1] = m[0][0] * m[2][2] - m[0][2] * m[2][0];
+    cof[1][2] = -(m[0][0] * m[2][1] - m[0][1] * m[2][0]);
+    cof[2][0] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+    cof[2][1] = -(m[0][0] * m[1][2] - m[0][2] * m[1][0]);
+    cof[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+    double det = m[0][0] * cof[0][0] + m[0][1] * cof[0][1] + m[0][2] * cof[0][2];
+    double k = std::pow(std::fabs(det), 2.0 / 3.0);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) G[r * 3 + c] = (float)(cof[r][c] / k);
+}
+
+// Renders the batch keeping (triangle << 1 | sub) per z slot.
+static int shading_visibility(sloth_shading* sh)
+{
+    sloth_batch* b = sh->b;
+    Job& j = b->job;
+    int rc;
+    if ((rc = j.prepare(b->stream))) return rc;
+    if ((rc = j.launch(0, (uint32_t)j.frames.size(), b->stream, sh->d_vis.as<int32_t>(), nullptr, true))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    bool replanned = false;
+    if ((rc = j.finish(0, (uint32_t)j.frames.size(), b->stream, sh->d_vis.as<int32_t>(), &replanned, true)))
+        return rc;
+    if (replanned) drop_graph(b);
+    b->ran = true;
+    b->checked = true;
+    return SLOTH_OK;
+}
+
+extern "C" {
+
+int sloth_shading_create(sloth_batch* b, const uint32_t* faces, uint32_t n_tris, uint32_t n_verts,
+                         sloth_shading** out)
+{
+    ARG_CHECK(out, "null output");
+    *out = nullptr;
+    ARG_CHECK(b && (faces || n_tris == 0), "null argument");
+    Job& j = b->job;
+    for (const Draw& d : j.draws) ARG_CHECK(d.n == n_tris, "every draw of the batch must use a mesh of n_tris triangles");
+    for (uint64_t i = 0; i < (uint64_t)n_tris * 3; ++i) ARG_CHECK(faces[i] < n_verts, "face vertex index out of range");
+    int rc = ensure_device();
+    if (rc) return rc;
+    sloth_shading* sh = new (std::nothrow) sloth_shading();
+    if (!sh) return fail(SLOTH_ENOMEM, "host allocation failed");
+    sh->b = b;
+    sh->n_tris = n_tris;
+    sh->n_verts = n_verts;
+    sh->n_slots = j.n_z;
+    // vertex -> (triangle, corner) adjacency, ascending
+    std::vector<uint32_t> adj_off(n_verts + 1, 0), adj((size_t)n_tris * 3);
+    for (uint64_t i = 0; i < (uint64_t)n_tris * 3; ++i) adj_off[faces[i] + 1]++;
+    for (uint32_t v = 0; v < n_verts; ++v) adj_off[v + 1] += adj_off[v];
+    {
+        std::vector<uint32_t> fill(adj_off.begin(), adj_off.end() - 1);
+        for (uint64_t i = 0; i < (uint64_t)n_tris * 3; ++i) adj[fill[faces[i]]++] = (uint32_t)i;
+    }
+    std::vector<float> G(j.draws.size() * 9);
+    for (size_t d = 0; d < j.draws.size(); ++d) normal_map(j.draws[d].M, &G[d * 9]);
+    const uint32_t nb = shade_blocks(sh->n_slots);
+    sh->partial.resize((size_t)nb * 4);
+    if ((rc = sh->d_faces.reserve(std::max<size_t>(4, (size_t)n_tris * 12))) ||
+        (rc = sh->d_vis.reserve(std::max<size_t>(4, sh->n_slots * 4))) ||
+        (rc = sh->d_G.reserve(std::max<size_t>(4, G.size() * 4))) ||
+        (rc = sh->d_vcolor.reserve(std::max<size_t>(4, (size_t)n_verts * 12))) ||
+        (rc = sh->d_vnormal.reserve(std::max<size_t>(4, (size_t)n_verts * 12))) ||
+        (rc = sh->d_light.reserve(16)) ||
+        (rc = sh->d_partial.reserve(std::max<size_t>(8, (size_t)nb * 32))) ||
+        (rc = sh->d_tri_acc.reserve(std::max<size_t>(8, (size_t)n_tris * 18 * 8))) ||
+        (rc = sh->d_adj_off.reserve((size_t)(n_verts + 1) * 4)) ||
+        (rc = sh->d_adj.reserve(std::max<size_t>(4, adj.size() * 4))) ||
+        (rc = sh->d_dcolor.reserve(std::max<size_t>(4, (size_t)n_verts * 12))) ||
+        (rc = sh->d_dnormal.reserve(std::max<size_t>(4, (size_t)n_verts * 12)))) {
+        delete sh;
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (n_tris) e = hipMemcpy(sh->d_faces.p, faces, (size_t)n_tris * 12, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !G.empty()) e = hipMemcpy(sh->d_G.p, G.data(), G.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(sh->d_adj_off.p, adj_off.data(), adj_off.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !adj.empty()) e = hipMemcpy(sh->d_adj.p, adj.data(), adj.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete sh;
+        return fail(SLOTH_EHIP, "shading upload: %s", hipGetErrorString(e));
+    }
+    if ((rc = shading_visibility(sh))) {
+        delete sh;
+        return rc;
+    }
+    *out = sh;
+    return SLOTH_OK;
+}
+
+int sloth_shading_visibility(sloth_shading* sh)
+{
+    ARG_CHECK(sh, "null shading");
+    Job& j = sh->b->job;
+    std::vector<float> G(j.draws.size() * 9);
+    for (size_t d = 0; d < j.draws.size(); ++d) normal_map(j.draws[d].M, &G[d * 9]);
+    if (!G.empty()) HIP_TRY(hipMemcpy(sh->d_G.p, G.data(), G.size() * 4, hipMemcpyHostToDevice));
+    return shading_visibility(sh);
+}
+
+int sloth_shading_slots(const sloth_shading* sh, uint64_t* n_slots)
+{
+    ARG_CHECK(sh && n_slots, "null argument");
+    *n_slots = sh->n_slots;
+    return SLOTH_OK;
+}
+
+int sloth_shading_set_target(sloth_shading* sh, const float* target)
+{
+    ARG_CHECK(sh, "null shading");
+    if (!target) {
+        sh->has_target = false;
+        return SLOTH_OK;
+    }
+    int rc = sh->d_target.reserve(std::max<size_t>(4, sh->n_slots * 12));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(sh->d_target.p, target, sh->n_slots * 12, hipMemcpyHostToDevice));
+    sh->has_target = true;
+    return SLOTH_OK;
+}
+
+static int shading_reduce(sloth_shading* sh, double* loss, double* dlight)
+{
+    const uint32_t nb = shade_blocks(sh->n_slots);
+    if (nb) HIP_TRY(hipMemcpy(sh->partial.data(), sh->d_partial.p, (size_t)nb * 32, hipMemcpyDeviceToHost));
+    double acc[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < nb; ++i)
+        for (int k = 0; k < 4; ++k) acc[k] += sh->partial[(size_t)i * 4 + k];
+    if (loss) *loss = acc[0];
+    if (dlight)
+        for (int k = 0; k < 3; ++k) dlight[k] = acc[k + 1];
+    return SLOTH_OK;
+}
+
+int sloth_shading_forward(sloth_shading* sh, const float* vcolor, const float* vnormal, const float* light,
+                          float* out_rgb, double* loss)
+{
+    ARG_CHECK(sh && vcolor && vnormal && light, "null argument");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = sh->b->stream;
+    if (sh->n_verts) {
+        HIP_TRY(hipMemcpyAsync(sh->d_vcolor.p, vcolor, (size_t)sh->n_verts * 12, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(sh->d_vnormal.p, vnormal, (size_t)sh->n_verts * 12, hipMemcpyHostToDevice, s));
+    }
+    HIP_TRY(hipMemcpyAsync(sh->d_light.p, light, 12, hipMemcpyHostToDevice, s));
+    sh->has_params = true;
+    float* d_out = nullptr;
+    if (out_rgb) {
+        if ((rc = sh->d_out.reserve(std::max<size_t>(4, sh->n_slots * 12)))) return rc;
+        d_out = sh->d_out.as<float>();
+        HIP_TRY(hipMemsetAsync(sh->d_out.p, 0, sh->n_slots * 12, s));
+    }
+    if ((rc = launch_shade(shade_args(sh, sh->has_target, d_out), false, s))) return rc;
+    if (out_rgb) HIP_TRY(hipMemcpyAsync(out_rgb, sh->d_out.p, sh->n_slots * 12, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return shading_reduce(sh, loss, nullptr);
+}
+
+int sloth_shading_backward(sloth_shading* sh, float* d_vcolor, float* d_vnormal, float* d_light)
+{
+    ARG_CHECK(sh, "null shading");
+    if (!sh->has_params) return fail(SLOTH_ESTATE, "backward needs a forward call first");
+    if (!sh->has_target) return fail(SLOTH_ESTATE, "backward needs a target (sloth_shading_set_target)");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = sh->b->stream;
+    ShadeArgs a = shade_args(sh, true, nullptr);
+    if ((rc = launch_shade(a, true, s))) return rc;
+    if ((rc = launch_shade_gather(a, sh->d_adj_off.as<uint32_t>(), sh->d_adj.as<uint32_t>(), sh->d_dcolor.as<float>(),
+                                  sh->d_dnormal.as<float>(), s)))
+        return rc;
+    if (d_vcolor && sh->n_verts)
+        HIP_TRY(hipMemcpyAsync(d_vcolor, sh->d_dcolor.p, (size_t)sh->n_verts * 12, hipMemcpyDeviceToHost, s));
+    if (d_vnormal && sh->n_verts)
+        HIP_TRY(hipMemcpyAsync(d_vnormal, sh->d_dnormal.p, (size_t)sh->n_verts * 12, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double dl[3];
+    if ((rc = shading_reduce(sh, nullptr, dl))) return rc;
+    if (d_light)
+        for (int k = 0; k < 3; ++k) d_light[k] = (float)dl[k];
+    return SLOTH_OK;
+}
+
+int sloth_shading_time(sloth_shading* sh, uint32_t steps, uint32_t warmup, float* ms)
+{
+    ARG_CHECK(sh && ms, "null argument");
+    if (!sh->has_params || !sh->has_target) return fail(SLOTH_ESTATE, "timing needs parameters and a target");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = sh->b->stream;
+    ShadeArgs fwd = shade_args(sh, true, nullptr), bwd = fwd;
+    auto step = [&]() -> int {
+        int r;
+        if ((r = launch_shade(fwd, false, s))) return r;
+        if ((r = launch_shade(bwd, true, s))) return r;
+        return launch_shade_gather(bwd, sh->d_adj_off.as<uint32_t>(), sh->d_adj.as<uint32_t>(),
+                                   sh->d_dcolor.as<float>(), sh->d_dnormal.as<float>(), s);
+    };
+    for (uint32_t i = 0; i < warmup; ++i)
+        if ((rc = step())) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+    for (uint32_t i = 0; i < steps && rc == SLOTH_OK; ++i) rc = step();
+    hipError_t e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SLOTH_EHIP, "shading timing: %s", hipGetErrorString(e));
+    *ms = t;
+    return SLOTH_OK;
+}
+
+int sloth_shading_destroy(sloth_shading* sh)
+{
+    if (sh && sh->b && sh->b->stream) (void)hipStreamSynchronize(sh->b->stream);
+    delete sh;
     return SLOTH_OK;
 }
 

@@ -126,7 +126,8 @@ __global__ void __launch_bounds__(256) k_resolve_global(const Draw* __restrict__
                                                         uint32_t f_hi, uint64_t cell_base, uint64_t n_cells,
                                                         const uint64_t* __restrict__ keys,
                                                         const int32_t* __restrict__ nl, uint32_t* __restrict__ cells,
-                                                        float* __restrict__ zbuf, int32_t* __restrict__ win)
+                                                        float* __restrict__ zbuf, int32_t* __restrict__ win,
+                                                        uint32_t win_raw)
 {
     uint64_t t_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t_id >= n_cells) return;
@@ -155,7 +156,7 @@ __global__ void __launch_bounds__(256) k_resolve_global(const Draw* __restrict__
             uint64_t tm = (((uint64_t)tri << 32) | ((uint64_t)py << 16) | px) + 1;
             best_time = tm;
             best_cell = cell;
-            wt = (int32_t)tri;
+            wt = win_raw ? (int32_t)(uint32_t)k : (int32_t)tri;   // raw: triangle << 1 | sub
         }
         zbuf[fr.z_off + c] = z;
         if (win) win[fr.z_off + c] = wt;

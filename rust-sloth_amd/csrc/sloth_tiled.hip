@@ -460,7 +460,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
     uint32_t f_hi, uint64_t tile_base, int tws, int ths, const uint32_t* __restrict__ cursor,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ pool, const int32_t* __restrict__ nl,
     const uint4* __restrict__ careful_box, uint32_t* __restrict__ cells, float* __restrict__ zbuf,
-    int32_t* __restrict__ win)
+    int32_t* __restrict__ win, uint32_t win_raw)
 {
     extern __shared__ uint64_t smem[];
     const int sws = tws - 1;
@@ -529,7 +529,11 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
         const uint64_t id = (uint64_t)row * W + (uint64_t)col;
         *(uint2*)(fc + id) = make_uint2(c0, c1);
         *(float2*)(fz + id) = make_float2(z0, F32_MAX);
-        if (win) *(int2*)(win + fr.z_off + id) = make_int2(wt, -1);
+        if (win) {
+            // win_raw keeps the key's low word (triangle << 1 | sub) for the shading pass
+            const int32_t wv = (k == KEY_EMPTY) ? -1 : (win_raw ? (int32_t)(uint32_t)k : wt);
+            *(int2*)(win + fr.z_off + id) = make_int2(wv, -1);
+        }
     };
     auto slot_valid = [&](uint32_t s) {
         return g.tr0 + (int)(s >> sws) < g.tr1 && g.tc0 + 2 * (int)(s & ((1u << sws) - 1)) < g.tc1;
@@ -811,7 +815,7 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
 
 int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
                       const Draw* d_draws, const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells,
-                      float* z, int32_t* win, hipStream_t s, std::vector<hipEvent_t>* ev)
+                      float* z, int32_t* win, bool win_raw, hipStream_t s, std::vector<hipEvent_t>* ev)
 {
     if (!sized) return fail(SLOTH_ESTATE, "tile lists not sized");
     int rc;
@@ -829,7 +833,7 @@ int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>&
         hipLaunchKernelGGL(k_tile_raster, dim3((unsigned)(t_hi - t_lo)), dim3(RASTER_BLOCK), raster_lds(), s, d_draws,
                            d_ramps, d_frames, f_lo, f_hi, t_lo, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
                            d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_careful_box.as<uint4>(),
-                           cells, z, win);
+                           cells, z, win, win_raw ? 1u : 0u);
         HIP_TRY(hipGetLastError());
     }
     if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));

@@ -90,6 +90,14 @@ SIGNATURES = {
     "sloth_batch_profile": (C.c_int, [_vp, _fp, C.POINTER(C.c_char_p), _u32p]),
     "sloth_batch_time": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int, _fp]),
     "sloth_batch_destroy": (C.c_int, [_vp]),
+    "sloth_shading_create": (C.c_int, [_vp, _u32p, C.c_uint32, C.c_uint32, _vpp]),
+    "sloth_shading_visibility": (C.c_int, [_vp]),
+    "sloth_shading_slots": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    "sloth_shading_set_target": (C.c_int, [_vp, _fp]),
+    "sloth_shading_forward": (C.c_int, [_vp, _fp, _fp, _fp, _fp, C.POINTER(C.c_double)]),
+    "sloth_shading_backward": (C.c_int, [_vp, _fp, _fp, _fp]),
+    "sloth_shading_time": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _fp]),
+    "sloth_shading_destroy": (C.c_int, [_vp]),
     "sloth_screen_transform": (C.c_int, [C.c_uint32, C.c_uint32, _fp, C.c_uint32, _fp]),
     "sloth_mat4_mul": (C.c_int, [_fp, _fp, _fp]),
 }
@@ -430,4 +438,56 @@ class Batch:
         h = getattr(self, "_h", None)
         if h is not None and _lib is not None:
             _lib.sloth_batch_destroy(h)
+            self._h = None
+
+
+class Shading:
+    """Differentiable shading of a batch's visible fragments (per-vertex
+    colours and normals, light direction; visibility fixed).  The forward and
+    backward pair of the C-ABI (sloth_shading_forward / _backward)."""
+
+    def __init__(self, batch: Batch, faces, n_verts: int):
+        L = load_library()
+        self.batch = batch
+        self.faces = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        self.n_verts = int(n_verts)
+        h = C.c_void_p()
+        _check(L.sloth_shading_create(batch._h, self.faces.ctypes.data_as(_u32p), self.faces.shape[0],
+                                      self.n_verts, C.byref(h)))
+        self._h = h
+        n = C.c_uint64()
+        _check(L.sloth_shading_slots(h, C.byref(n)))
+        self.n_slots = int(n.value)
+
+    def set_target(self, target) -> None:
+        t = None if target is None else np.ascontiguousarray(target, np.float32).reshape(self.n_slots, 3)
+        _check(load_library().sloth_shading_set_target(self._h, _fptr(t) if t is not None else None))
+        self._target = t
+
+    def forward(self, vcolor, vnormal, light, want_out: bool = False):
+        vc = np.ascontiguousarray(vcolor, np.float32).reshape(self.n_verts, 3)
+        vn = np.ascontiguousarray(vnormal, np.float32).reshape(self.n_verts, 3)
+        li = np.ascontiguousarray(light, np.float32).reshape(3)
+        out = np.empty((self.n_slots, 3), np.float32) if want_out else None
+        loss = C.c_double()
+        _check(load_library().sloth_shading_forward(self._h, _fptr(vc), _fptr(vn), _fptr(li),
+                                                    _fptr(out) if want_out else None, C.byref(loss)))
+        return float(loss.value), out
+
+    def backward(self):
+        dc = np.empty((self.n_verts, 3), np.float32)
+        dn = np.empty((self.n_verts, 3), np.float32)
+        dl = np.empty(3, np.float32)
+        _check(load_library().sloth_shading_backward(self._h, _fptr(dc), _fptr(dn), _fptr(dl)))
+        return dc, dn, dl
+
+    def time(self, steps: int, warmup: int) -> float:
+        ms = C.c_float()
+        _check(load_library().sloth_shading_time(self._h, steps, warmup, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.sloth_shading_destroy(h)
             self._h = None
