@@ -191,6 +191,17 @@ int sloth_batch_profile(sloth_batch *b, float *out_ms, const char **names, uint3
  * milliseconds of the timed renders. */
 int sloth_batch_time(sloth_batch *b, uint32_t steps, uint32_t warmup, int mode, float *ms);
 
+/* Tile size of the batch's even-width frames, in cells (width a power of two
+ * in 8..128, height in 4..64).  Takes effect at the next render. */
+int sloth_batch_set_tile(sloth_batch *b, uint32_t tile_w, uint32_t tile_h);
+
+/* Renders the batch with each candidate tile size (32..128 x 8..32 cells),
+ * timed with HIP events, and keeps the fastest; the choice is remembered for
+ * batches of the same class (frame sizes and triangle count) in this process
+ * and reused without timing.  Writes the chosen size to tile_w / tile_h (may
+ * be NULL). */
+int sloth_batch_autotune(sloth_batch *b, uint32_t *tile_w, uint32_t *tile_h);
+
 int sloth_batch_destroy(sloth_batch *b);
 
 /* ---- differentiable shading (no reference counterpart: the reference shades

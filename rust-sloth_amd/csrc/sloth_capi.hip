@@ -361,7 +361,8 @@ struct Job {
         if (f_lo >= f_hi) return SLOTH_OK;
         if (path == PATH_TILED)
             return tiled.launch(frames, draws, d_frames.as<Frame>(), d_draws.as<Draw>(), d_ramps.as<Ramp>(), f_lo,
-                                f_hi, d_cells.as<uint32_t>(), d_z.as<float>(), win, win_raw, s, ev);
+                                f_hi, d_cells.as<uint32_t>(), d_z.as<float>(), win, win_raw, (uint32_t)ramps.size(), s,
+                                ev);
         return launch_global(f_lo, f_hi, s, win, win_raw, ev);
     }
 
@@ -960,6 +961,91 @@ int sloth_batch_time(sloth_batch* b, uint32_t steps, uint32_t warmup, int mode, 
         drop_graph(b);   // tile lists were re-sized: the timed renders were redone; time again
     }
     return fail(SLOTH_ESTATE, "tile lists kept re-sizing while timing");
+}
+
+int sloth_batch_set_tile(sloth_batch* b, uint32_t tile_w, uint32_t tile_h)
+{
+    ARG_CHECK(b, "null batch");
+    int rc = b->job.tiled.set_tile(tile_w, tile_h);
+    if (rc) return rc;
+    if (!b->job.tiled.sized) {
+        if ((rc = b->job.layout()) || (rc = b->job.upload(b->stream))) return rc;
+        drop_graph(b);
+        b->ran = false;
+    }
+    return SLOTH_OK;
+}
+
+}  // extern "C"
+
+// Autotuner cache: class of a batch -> best tile size.
+#include <map>
+#include <mutex>
+#include <tuple>
+
+static std::mutex g_tune_mu;
+static std::map<std::tuple<uint64_t, uint64_t, uint32_t>, std::pair<uint32_t, uint32_t>> g_tune;
+
+static std::tuple<uint64_t, uint64_t, uint32_t> tune_class(const Job& j)
+{
+    // frame sizes (order-independent hash), total cells, log2 of the triangle count
+    uint64_t h = 1469598103934665603ull, tris = 0;
+    std::vector<uint64_t> sz;
+    for (const Frame& f : j.frames) sz.push_back(((uint64_t)f.W << 32) | f.H);
+    std::sort(sz.begin(), sz.end());
+    for (uint64_t v : sz) h = (h ^ v) * 1099511628211ull;
+    for (const Draw& d : j.draws) tris += d.n;
+    uint32_t lg = 0;
+    while ((1ull << lg) < tris + 1) ++lg;
+    return std::make_tuple(h, j.n_cells, lg);
+}
+
+extern "C" {
+
+int sloth_batch_autotune(sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h)
+{
+    ARG_CHECK(b, "null batch");
+    int rc = ensure_device();
+    if (rc) return rc;
+    Job& j = b->job;
+    if (j.path != PATH_TILED) {
+        if (tile_w) *tile_w = 0;
+        if (tile_h) *tile_h = 0;
+        return SLOTH_OK;   // the general path has no tiles
+    }
+    const auto key = tune_class(j);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            if ((rc = sloth_batch_set_tile(b, it->second.first, it->second.second))) return rc;
+            if (tile_w) *tile_w = it->second.first;
+            if (tile_h) *tile_h = it->second.second;
+            return SLOTH_OK;
+        }
+    }
+    static const uint32_t cand[][2] = {{64, 16}, {64, 8}, {64, 32}, {128, 8}, {128, 16}, {128, 32},
+                                       {32, 16}, {32, 32}, {32, 8}};
+    float best = 1e30f;
+    uint32_t bw = j.tiled.TW, bh = j.tiled.TH;
+    for (const auto& c : cand) {
+        if ((rc = sloth_batch_set_tile(b, c[0], c[1]))) return rc;
+        float ms = 0.0f;
+        if ((rc = sloth_batch_time(b, 2, 1, 0, &ms))) return rc;
+        if (ms < best) {
+            best = ms;
+            bw = c[0];
+            bh = c[1];
+        }
+    }
+    if ((rc = sloth_batch_set_tile(b, bw, bh))) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = std::make_pair(bw, bh);
+    }
+    if (tile_w) *tile_w = bw;
+    if (tile_h) *tile_h = bh;
+    return SLOTH_OK;
 }
 
 int sloth_batch_destroy(sloth_batch* b)

@@ -65,7 +65,7 @@ struct TiledPlan {
     // gets 5 events recorded around the 4 stages.
     int launch(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
                const Draw* d_draws, const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells, float* z,
-               int32_t* win, bool win_raw, hipStream_t s, std::vector<hipEvent_t>* ev);
+               int32_t* win, bool win_raw, uint32_t n_ramps, hipStream_t s, std::vector<hipEvent_t>* ev);
 
     // Reads the overflow flags of frames [0, nf) (synchronises the stream).
     int overflowed(uint32_t nf, hipStream_t s, bool& ovf);

@@ -393,6 +393,7 @@ __global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws,
 constexpr uint32_t CACHE_CAP = 256;   // list length up to which the fast path runs
 constexpr uint32_t HASH_CAP = 512;    // open-addressing table: triangle -> cache row
 constexpr uint32_t CACHE_WORDS = 12;
+constexpr uint32_t LDS_RAMPS = 4;     // shaders staged in LDS per tile
 
 __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws, const Frame& fr, uint32_t tri)
 {
@@ -460,9 +461,14 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
     uint32_t f_hi, uint64_t tile_base, int tws, int ths, const uint32_t* __restrict__ cursor,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ pool, const int32_t* __restrict__ nl,
     const uint4* __restrict__ careful_box, uint32_t* __restrict__ cells, float* __restrict__ zbuf,
-    int32_t* __restrict__ win, uint32_t win_raw)
+    int32_t* __restrict__ win, uint32_t win_raw, uint32_t n_ramps)
 {
     extern __shared__ uint64_t smem[];
+    // the job's first LDS_RAMPS shaders, so that a glyph lookup reads LDS
+    __shared__ Ramp s_ramp[LDS_RAMPS];
+    for (uint32_t i = threadIdx.x; i < min(n_ramps, LDS_RAMPS) * (uint32_t)sizeof(Ramp) / 4; i += blockDim.x)
+        ((uint32_t*)s_ramp)[i] = ((const uint32_t*)ramps)[i];
+    auto ramp_of = [&](uint32_t r) -> const Ramp& { return r < LDS_RAMPS ? s_ramp[r] : ramps[r]; };
     const int sws = tws - 1;
     const uint32_t S = 1u << (sws + ths);
     uint64_t* keys = smem;
@@ -602,7 +608,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
                 z = q1.z + q2.y * (w1 * q1.w + w2 * q2.x);
                 const float sh = q2.z * (w0 + w1 + w2);
                 const uint32_t cw = __float_as_uint(q2.w);
-                cell = (uint32_t)ramp_glyph(ramps[cw & 0xFFu], sh) | (cw & 0xFFFFFF00u);
+                cell = (uint32_t)ramp_glyph(ramp_of(cw & 0xFFu), sh) | (cw & 0xFFFFFF00u);
             }
             store(s, k, cell, z);
         }
@@ -647,7 +653,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
                     color = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24);
                     have_nz = true;
                 }
-                st_cell[slot] = (uint32_t)ramp_glyph(ramps[dr.ramp], na * (w0 + w1 + w2)) | color;
+                st_cell[slot] = (uint32_t)ramp_glyph(ramp_of(dr.ramp), na * (w0 + w1 + w2)) | color;
                 st_z[slot] = z;
             });
         }
@@ -815,7 +821,8 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
 
 int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
                       const Draw* d_draws, const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells,
-                      float* z, int32_t* win, bool win_raw, hipStream_t s, std::vector<hipEvent_t>* ev)
+                      float* z, int32_t* win, bool win_raw, uint32_t n_ramps, hipStream_t s,
+                      std::vector<hipEvent_t>* ev)
 {
     if (!sized) return fail(SLOTH_ESTATE, "tile lists not sized");
     int rc;
@@ -833,7 +840,7 @@ int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>&
         hipLaunchKernelGGL(k_tile_raster, dim3((unsigned)(t_hi - t_lo)), dim3(RASTER_BLOCK), raster_lds(), s, d_draws,
                            d_ramps, d_frames, f_lo, f_hi, t_lo, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
                            d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_careful_box.as<uint4>(),
-                           cells, z, win, win_raw ? 1u : 0u);
+                           cells, z, win, win_raw ? 1u : 0u, n_ramps);
         HIP_TRY(hipGetLastError());
     }
     if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));

@@ -89,6 +89,8 @@ SIGNATURES = {
     "sloth_batch_device_frame": (C.c_int, [_vp, C.c_uint32, _vpp, _vpp]),
     "sloth_batch_profile": (C.c_int, [_vp, _fp, C.POINTER(C.c_char_p), _u32p]),
     "sloth_batch_time": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int, _fp]),
+    "sloth_batch_set_tile": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    "sloth_batch_autotune": (C.c_int, [_vp, _u32p, _u32p]),
     "sloth_batch_destroy": (C.c_int, [_vp]),
     "sloth_shading_create": (C.c_int, [_vp, _u32p, C.c_uint32, C.c_uint32, _vpp]),
     "sloth_shading_visibility": (C.c_int, [_vp]),
@@ -425,6 +427,15 @@ class Batch:
         ms = C.c_float()
         _check(load_library().sloth_batch_time(self._h, steps, warmup, int(per_frame), C.byref(ms)))
         return float(ms.value)
+
+    def set_tile(self, tile_w: int, tile_h: int) -> None:
+        _check(load_library().sloth_batch_set_tile(self._h, tile_w, tile_h))
+
+    def autotune(self) -> tuple:
+        """Times the candidate tile sizes and keeps the fastest; returns it."""
+        w, h = C.c_uint32(), C.c_uint32()
+        _check(load_library().sloth_batch_autotune(self._h, C.byref(w), C.byref(h)))
+        return int(w.value), int(h.value)
 
     def profile(self) -> dict:
         cap = 16

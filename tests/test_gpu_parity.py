@@ -256,3 +256,53 @@ def test_empty_mesh_and_frame(sloth):
     I = np.eye(4, dtype=np.float32).reshape(16)
     _check(sloth, "empty mesh", W, H, I, [(np.zeros((0, 3, 3), np.float32), np.zeros((0, 3), np.uint8), I, None)])
     _check(sloth, "no draws", W, H, I, [])
+
+
+@pytest.mark.parametrize("tile", [(32, 8), (128, 32), (8, 4), (64, 64)])
+def test_tile_sizes_agree(sloth, tile, raster_path):
+    """Every tile size gives the oracle's frames (the wrap and newline cells
+    land in different tiles for different sizes)."""
+    from sloth_amd import meshgen
+
+    if raster_path == "global":
+        pytest.skip("tile sizes only exist on the tiled path")
+    m = meshgen.displaced_icosphere(3, seed=9)
+    frames, draws, plan = [], [], []
+    sm = sloth.SimpleMesh.from_mesh(m)
+    for f, (W, H) in enumerate([(200, 64), (130, 40), (64, 30)]):
+        U = meshgen.screen_transform(W, H, [m.bbox_max()])
+        R = meshgen.rotation_euler(0.3 * f, 1.0 + f, 0.2)
+        frames.append(sloth.FrameSpec(W, H, U, True))
+        draws.append(sloth.DrawSpec(f, sm, R))
+        plan.append((W, H, U, R))
+    b = sloth.Batch(frames, draws)
+    b.set_tile(*tile)
+    say(f"case tile {tile}")
+    b.run()
+    b.sync()
+    for f, (W, H, U, R) in enumerate(plan):
+        ref = oracle_render(W, H, U, [(m.xyz, m.rgb, R, None)])
+        cells, z, win = b.read_frame(f, want_winners=True)
+        assert_frame_equal(cells, z, win, ref, W)
+
+
+def test_autotune_keeps_results(sloth, raster_path):
+    from sloth_amd import meshgen
+
+    if raster_path == "global":
+        pytest.skip("tile sizes only exist on the tiled path")
+    m = meshgen.displaced_icosphere(4, seed=4)
+    W, H = 320, 120
+    U = meshgen.screen_transform(W, H, [m.bbox_max()])
+    sm = sloth.SimpleMesh.from_mesh(m)
+    Rs = [meshgen.rotation_euler(0.0, 0.5 * k, 0.0) for k in range(4)]
+    b = sloth.Batch([sloth.FrameSpec(W, H, U, True) for _ in Rs], [sloth.DrawSpec(k, sm, R) for k, R in enumerate(Rs)])
+    tw, th = b.autotune()
+    say(f"case autotune -> {tw}x{th}")
+    assert tw in (32, 64, 128) and th in (8, 16, 32)
+    b.run()
+    b.sync()
+    for k, R in enumerate(Rs):
+        ref = oracle_render(W, H, U, [(m.xyz, m.rgb, R, None)])
+        cells, z, win = b.read_frame(k, want_winners=True)
+        assert_frame_equal(cells, z, win, ref, W)

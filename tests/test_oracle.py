@@ -246,3 +246,83 @@ def test_datajs_structure():
         mine = {((c >> 8) & 255, (c >> 16) & 255, (c >> 24) & 255) for c in np.unique(f.cells)}
         assert mine <= cols | {(0, 0, 0)}
     assert agree / total > 0.98
+
+
+# --- FP64 shading model (the added differentiable form) --------------------
+
+def _shading_case(seed=2):
+    from sloth_amd import meshgen
+
+    m = meshgen.uv_sphere(16, 8, 1.0, seed=seed)
+    W, H = 60, 30
+    U = oracle.update(W, H, [m.bbox_max()])
+    R = oracle.rotation_euler(0.2, 2.2, 0.1)
+    fr = oracle.render(W, H, U, m.xyz, m.rgb, R)
+    rng = np.random.default_rng(seed)
+    vc = rng.uniform(0, 1, (m.verts.shape[0], 3))
+    vn = m.verts / np.linalg.norm(m.verts, axis=1, keepdims=True)
+    L = np.array([0.3, 0.5, -0.8])
+    target = rng.uniform(0, 1, (W * H, 3)).astype(np.float32)
+    return m, oracle.mat4_mul(U, R), fr, vc.astype(np.float64), vn.astype(np.float64), L, target
+
+
+def test_shade_adjoint_matches_central_differences():
+    """The loss is quadratic in each parameter group, so central differences
+    are exact up to float64 rounding."""
+    m, M, fr, vc, vn, L, target = _shading_case()
+    loss, _, (gc, gn, gl) = oracle.shade_f64(fr, M, m.xyz, m.faces, vc, vn, L, target, want_grad=True)
+    assert loss > 0
+    h = 1e-3
+    rng = np.random.default_rng(0)
+    used = np.unique(m.faces[fr.win[fr.win >= 0]])
+    for v in rng.choice(used, 6, replace=False):
+        for ch in range(3):
+            for base, grad, slot in ((vc, gc, 0), (vn, gn, 1)):
+                p, q = base.copy(), base.copy()
+                p[v, ch] += h
+                q[v, ch] -= h
+                args_p = [vc, vn, L]
+                args_q = [vc, vn, L]
+                args_p[slot], args_q[slot] = p, q
+                fd = (oracle.shade_f64(fr, M, m.xyz, m.faces, *args_p, target)[0]
+                      - oracle.shade_f64(fr, M, m.xyz, m.faces, *args_q, target)[0]) / (2 * h)
+                assert abs(fd - grad[v, ch]) <= 1e-7 * max(1.0, abs(fd))
+    for ch in range(3):
+        p, q = L.copy(), L.copy()
+        p[ch] += h
+        q[ch] -= h
+        fd = (oracle.shade_f64(fr, M, m.xyz, m.faces, vc, vn, p, target)[0]
+              - oracle.shade_f64(fr, M, m.xyz, m.faces, vc, vn, q, target)[0]) / (2 * h)
+        assert abs(fd - gl[ch]) <= 1e-7 * max(1.0, abs(fd))
+
+
+def test_shade_reduces_to_flat_shade():
+    """Per-corner normals = the face's unit normal, L = (0,0,1), colours = 1:
+    the intensity equals the reference's nz_unit * a * (w0 + w1 + w2)."""
+    from sloth_amd import meshgen
+
+    base = meshgen.uv_sphere(16, 8, seed=3)
+    n = base.n
+    faces = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    e1 = base.xyz[:, 1].astype(np.float64) - base.xyz[:, 0]
+    e2 = base.xyz[:, 2].astype(np.float64) - base.xyz[:, 0]
+    fn = np.cross(e1, e2)
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    W, H = 60, 30
+    U = oracle.update(W, H, [base.bbox_max()])
+    R = oracle.rotation_euler(0.1, 1.9, 0.0)
+    fr = oracle.render(W, H, U, base.xyz, base.rgb, R)
+    M = oracle.mat4_mul(U, R)
+    _, out, _ = oracle.shade_f64(fr, M, base.xyz, faces, np.ones((3 * n, 3)), np.repeat(fn, 3, 0),
+                                 np.array([0.0, 0.0, 1.0]), want_out=True)
+    for idx in np.nonzero(fr.win >= 0)[0]:
+        T = fr.win[idx]
+        t = oracle.tri_mul(M, np.concatenate([base.xyz[T], np.ones((3, 1), np.float32)], 1)).astype(np.float64)
+        x, y = fr.winxy[2 * idx], fr.winxy[2 * idx + 1]
+        o = (t[1, 0] - t[0, 0]) * (t[2, 1] - t[0, 1]) - (t[1, 1] - t[0, 1]) * (t[2, 0] - t[0, 0])
+        w = [(t[2, 0] - t[1, 0]) * (y - t[1, 1]) - (t[2, 1] - t[1, 1]) * (x - t[1, 0]),
+             (t[0, 0] - t[2, 0]) * (y - t[2, 1]) - (t[0, 1] - t[2, 1]) * (x - t[2, 0]),
+             (t[1, 0] - t[0, 0]) * (y - t[0, 1]) - (t[1, 1] - t[0, 1]) * (x - t[0, 0])]
+        n_s = np.cross(t[1, :3] - t[0, :3], t[2, :3] - t[0, :3])
+        flat = n_s[2] / np.linalg.norm(n_s) * sum(w) / o
+        assert abs(out[idx, 0] - flat) < 1e-6
