@@ -135,6 +135,19 @@ int sloth_context_read(sloth_context *ctx, uint32_t *cells, float *z, int32_t *w
 int sloth_context_device_buffers(sloth_context *ctx, uint32_t **d_cells, float **d_z, uint64_t *n_cells,
                                  uint64_t *n_z);
 
+/* Context::flush's byte streams (context.rs:50-92), encoded on the device:
+ *   SLOTH_ENCODE_TEXT  the no-colour flush (context.rs:58-61): every cell's
+ *                      char, then println!'s '\n'
+ *   SLOTH_ENCODE_HTML  the -j export (context.rs:79-87): per cell
+ *                      `<span style="color:rgb(R,G,B)">` then the char
+ * Chars are written as UTF-8 (a glyph byte is a code point below 256).  The
+ * ANSI truecolour flush (context.rs:62-77) is not provided: its bytes are
+ * crossterm's.  With out = NULL only *len is set; otherwise out must hold
+ * cap >= *len bytes.  Renders first if needed. */
+#define SLOTH_ENCODE_TEXT 0
+#define SLOTH_ENCODE_HTML 2
+int sloth_context_encode(sloth_context *ctx, int mode, char *out, uint64_t cap, uint64_t *len);
+
 /* ---- batches (many frames in one launch sequence; no reference
  * counterpart: the reference renders one frame per loop iteration,
  * main.rs:60-111) ---------------------------------------------------- */
@@ -175,6 +188,9 @@ int sloth_batch_sync(sloth_batch *b);
 
 /* Copies one frame's cells / z / winners to the host (NULL = skip); waits. */
 int sloth_batch_read_frame(sloth_batch *b, uint32_t frame, uint32_t *cells, float *z, int32_t *winners);
+
+/* sloth_context_encode for one frame of a batch that has been run. */
+int sloth_batch_encode_frame(sloth_batch *b, uint32_t frame, int mode, char *out, uint64_t cap, uint64_t *len);
 
 /* Device addresses of one frame's cells and z. */
 int sloth_batch_device_frame(sloth_batch *b, uint32_t frame, uint32_t **d_cells, float **d_z);

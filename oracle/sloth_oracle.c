@@ -23,6 +23,7 @@
 #include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
+#include <stdio.h>
 #include <string.h>
 
 #include "sloth_oracle.h"
@@ -527,4 +528,50 @@ double oracle_shade_f64(uint32_t W, uint32_t H, const float *M, const float *xyz
             for (int r = 0; r < 3; ++r) dlight[r] += dI * n[r];
     }
     return loss;
+}
+
+/* ------------------------------------------------------------------ */
+/* Context::flush byte streams (context.rs:50-92)                       */
+/* ------------------------------------------------------------------ */
+
+static size_t put_char_utf8(char *p, uint32_t g)
+{
+    if (g < 0x80u) {
+        if (p) p[0] = (char)g;
+        return 1;
+    }
+    if (p) {
+        p[0] = (char)(0xC0u | (g >> 6));
+        p[1] = (char)(0x80u | (g & 0x3Fu));
+    }
+    return 2;
+}
+
+/* mode 0: `let frame: String = frame_buffer.iter().map(|p| p.0).collect();
+ * println!("{}", frame);` (context.rs:58-61).  mode 2: for every cell
+ * print!("<span style=\"color:rgb({},{},{})\">{}", r, g, b, char)
+ * (context.rs:79-87).  Writes into out (if non-NULL, cap bytes) and returns
+ * the full length. */
+size_t oracle_encode(const uint32_t *cells, uint64_t n, int mode, char *out, size_t cap)
+{
+    size_t len = 0;
+    char tmp[64];
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t c = cells[i];
+        if (mode == 2) {
+            int k = snprintf(tmp, sizeof tmp, "<span style=\"color:rgb(%u,%u,%u)\">", (c >> 8) & 0xFFu,
+                             (c >> 16) & 0xFFu, c >> 24);
+            if (out && len + (size_t)k <= cap) memcpy(out + len, tmp, (size_t)k);
+            len += (size_t)k;
+        }
+        char u[2];
+        size_t k = put_char_utf8(u, c & 0xFFu);
+        if (out && len + k <= cap) memcpy(out + len, u, k);
+        len += k;
+    }
+    if (mode == 0) {
+        if (out && len < cap) out[len] = '\n';
+        len += 1;
+    }
+    return len;
 }

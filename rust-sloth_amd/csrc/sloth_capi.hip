@@ -17,6 +17,7 @@
 
 #include "sloth_hip.h"
 #include "sloth_host.h"
+#include "sloth_encode.h"
 #include "sloth_kernels.h"
 #include "sloth_shade.h"
 #include "sloth_tiled.h"
@@ -382,6 +383,45 @@ static hipStream_t make_stream()
     return s;
 }
 
+// Device scratch of the flush encoders.
+struct Encoder {
+    DevBuf d_off, d_block, d_total, d_out;
+
+    int run(const uint32_t* d_cells, uint64_t n, int mode, char* out, uint64_t cap, uint64_t* len, hipStream_t s)
+    {
+        if (mode != SLOTH_ENCODE_TEXT && mode != SLOTH_ENCODE_HTML)
+            return fail(SLOTH_EINVAL, "encode mode must be SLOTH_ENCODE_TEXT or SLOTH_ENCODE_HTML");
+        int rc;
+        const uint64_t nb = encode_blocks(n);
+        if ((rc = d_off.reserve(std::max<uint64_t>(8, n * 8))) || (rc = d_block.reserve(std::max<uint64_t>(8, nb * 8))) ||
+            (rc = d_total.reserve(8)))
+            return rc;
+        uint64_t total = 0;
+        if (n) {
+            if ((rc = launch_encode_size(d_cells, n, mode, d_off.as<uint64_t>(), d_block.as<uint64_t>(),
+                                         d_total.as<uint64_t>(), s)))
+                return rc;
+            HIP_TRY(hipMemcpyAsync(&total, d_total.p, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        const uint64_t full = total + (mode == SLOTH_ENCODE_TEXT ? 1 : 0);   // println!'s newline
+        if (len) *len = full;
+        if (!out) return SLOTH_OK;
+        if (cap < full) return fail(SLOTH_EINVAL, "encode buffer holds %llu bytes, %llu needed",
+                                    (unsigned long long)cap, (unsigned long long)full);
+        if (total) {
+            if ((rc = d_out.reserve(total))) return rc;
+            if ((rc = launch_encode_write(d_cells, n, mode, d_off.as<uint64_t>(), d_block.as<uint64_t>(),
+                                          d_out.as<uint8_t>(), s)))
+                return rc;
+            HIP_TRY(hipMemcpyAsync(out, d_out.p, total, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        if (mode == SLOTH_ENCODE_TEXT) out[total] = '\n';
+        return SLOTH_OK;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // contexts
 // ---------------------------------------------------------------------------
@@ -392,6 +432,7 @@ struct sloth_context {
     bool rendered = false;      // device buffers match the draw list
     bool win_valid = false;     // d_win matches the draw list
     bool checked = false;       // the last render was checked for tile-list overflow
+    Encoder enc;
     ~sloth_context()
     {
         if (stream) (void)hipStreamDestroy(stream);
@@ -411,6 +452,7 @@ struct sloth_batch {
     bool ran = false;
     std::vector<bool> win_valid;
     bool checked = false;           // the last render was checked for tile-list overflow
+    Encoder enc;
     ~sloth_batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -694,6 +736,14 @@ int sloth_context_read(sloth_context* ctx, uint32_t* cells, float* z, int32_t* w
     return SLOTH_OK;
 }
 
+int sloth_context_encode(sloth_context* ctx, int mode, char* out, uint64_t cap, uint64_t* len)
+{
+    ARG_CHECK(ctx, "null context");
+    int rc = sloth_context_read(ctx, nullptr, nullptr, nullptr);   // render + overflow check
+    if (rc) return rc;
+    return ctx->enc.run(ctx->job.d_cells.as<uint32_t>(), ctx->job.n_cells, mode, out, cap, len, ctx->stream);
+}
+
 int sloth_context_device_buffers(sloth_context* ctx, uint32_t** d_cells, float** d_z, uint64_t* n_cells,
                                  uint64_t* n_z)
 {
@@ -873,6 +923,18 @@ int sloth_batch_read_frame(sloth_batch* b, uint32_t frame, uint32_t* cells, floa
         HIP_TRY(hipMemcpyAsync(winners, b->job.d_win.as<int32_t>() + fr.z_off, 4 * WH, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return SLOTH_OK;
+}
+
+int sloth_batch_encode_frame(sloth_batch* b, uint32_t frame, int mode, char* out, uint64_t cap, uint64_t* len)
+{
+    ARG_CHECK(b, "null batch");
+    ARG_CHECK(frame < b->job.frames.size(), "frame index out of range");
+    if (!b->ran) return fail(SLOTH_ESTATE, "batch has not been run");
+    int rc = sloth_batch_sync(b);
+    if (rc) return rc;
+    const Frame& fr = b->job.frames[frame];
+    const uint64_t nc = (uint64_t)fr.W * fr.H + (fr.image ? fr.H : 0);
+    return b->enc.run(b->job.d_cells.as<uint32_t>() + fr.cell_off, nc, mode, out, cap, len, b->stream);
 }
 
 int sloth_batch_device_frame(sloth_batch* b, uint32_t frame, uint32_t** d_cells, float** d_z)

@@ -27,6 +27,15 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libsloth_hip.so")
 
 SLOTH_OK, SLOTH_EINVAL, SLOTH_ENOMEM, SLOTH_EHIP, SLOTH_ENODEV, SLOTH_ESTATE = range(6)
+ENCODE_TEXT, ENCODE_HTML = 0, 2
+
+
+def _encode(fn, *args) -> bytes:
+    n = C.c_uint64()
+    _check(fn(*args, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(max(1, n.value))
+    _check(fn(*args, buf, n.value, C.byref(n)))
+    return buf.raw[:n.value]
 _ERRNAMES = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ENODEV", 5: "ESTATE"}
 
 
@@ -76,6 +85,9 @@ SIGNATURES = {
     "sloth_draw_mesh": (C.c_int, [_vp, _vp, _fp, C.POINTER(sloth_ramp)]),
     "sloth_context_render": (C.c_int, [_vp, _vp]),
     "sloth_context_read": (C.c_int, [_vp, _u32p, _fp, _i32p]),
+    "sloth_context_encode": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "sloth_batch_encode_frame": (C.c_int, [_vp, C.c_uint32, C.c_int, C.c_char_p, C.c_uint64,
+                                           C.POINTER(C.c_uint64)]),
     "sloth_context_device_buffers": (C.c_int, [_vp, _vpp, _vpp, C.POINTER(C.c_uint64),
                                                C.POINTER(C.c_uint64)]),
     "sloth_batch_create": (C.c_int, [C.POINTER(sloth_frame_desc), C.c_uint32,
@@ -318,6 +330,10 @@ class Context:
         """The no-colour flush (context.rs:58-61) without the trailing newline."""
         return (self.read()[0] & 0xFF).astype(np.uint8).tobytes().decode("latin-1")
 
+    def encode(self, mode: int = ENCODE_TEXT) -> bytes:
+        """Context::flush's bytes (context.rs:50-92), encoded on the GPU."""
+        return _encode(load_library().sloth_context_encode, self._h, mode)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and _lib is not None:
@@ -414,6 +430,9 @@ class Batch:
             self._h, f, cells.ctypes.data_as(_u32p), _fptr(z),
             win.ctypes.data_as(_i32p) if want_winners else None))
         return cells, z, win
+
+    def encode_frame(self, f: int, mode: int = ENCODE_TEXT) -> bytes:
+        return _encode(load_library().sloth_batch_encode_frame, self._h, f, mode)
 
     def device_frame(self, f: int):
         c = C.c_void_p()

@@ -1,0 +1,63 @@
+"""Context::flush byte streams (context.rs:50-92) encoded on the GPU, against
+the oracle's restatement, byte for byte."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from _helpers import say
+
+pytestmark = pytest.mark.gpu
+
+
+def test_encode_sphere_frame(sloth):
+    from sloth_amd import meshgen
+
+    m, W, H, U, R = meshgen.workload_sphere_160x48()
+    ctx = sloth.Context(W, H, image=True)
+    ctx.utransform = U
+    ctx.clear()
+    sloth.draw_mesh(ctx, sloth.SimpleMesh.from_mesh(m), R)
+    ref = oracle.render(W, H, U, m.xyz, m.rgb, R)
+    say("case encode sphere 160x48")
+    assert ctx.encode(sloth.ENCODE_TEXT) == oracle.encode(ref.cells, 0)
+    assert ctx.encode(sloth.ENCODE_HTML) == oracle.encode(ref.cells, 2)
+    text = ctx.encode(sloth.ENCODE_TEXT).decode()
+    assert text.count("\n") == H + 1 and "@" in text
+
+
+def test_encode_non_ascii_glyphs(sloth):
+    """Glyph bytes >= 0x80 are chars U+0080..U+00FF, two UTF-8 bytes each."""
+    from sloth_amd import meshgen, Shader
+
+    m = meshgen.uv_sphere(20, 10, seed=2)
+    W, H = 90, 30
+    U = meshgen.screen_transform(W, H, [m.bbox_max()])
+    R = meshgen.rotation_euler(0.0, 2.5, 0.0)
+    sh = Shader((0.25, 0.5, 0.75, 1.0), "·°¤é", "ÿ")
+    ctx = sloth.Context(W, H, image=True)
+    ctx.utransform = U
+    ctx.clear()
+    sloth.draw_mesh(ctx, sloth.SimpleMesh.from_mesh(m), R, sh)
+    cells, _, _ = ctx.read()
+    say("case encode non-ascii")
+    for mode in (sloth.ENCODE_TEXT, sloth.ENCODE_HTML):
+        got = ctx.encode(mode)
+        assert got == oracle.encode(cells, mode)
+        got.decode("utf-8")
+
+
+def test_encode_1080p_batch_frame(sloth):
+    from sloth_amd import meshgen
+
+    m, W, H, U, Rs = meshgen.workload_sweep_1080p(4, level=5)
+    sm = sloth.SimpleMesh.from_mesh(m)
+    b = sloth.Batch([sloth.FrameSpec(W, H, U, True) for _ in Rs], [sloth.DrawSpec(k, sm, R) for k, R in enumerate(Rs)])
+    b.run()
+    b.sync()
+    say("case encode 1080p batch frames")
+    for f in (0, 3):
+        cells, _, _ = b.read_frame(f)
+        assert b.encode_frame(f, sloth.ENCODE_TEXT) == oracle.encode(cells, 0)
+        assert b.encode_frame(f, sloth.ENCODE_HTML) == oracle.encode(cells, 2)
