@@ -316,18 +316,12 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 // wave's min/max; CLS_FULL: the whole domain.  The lanes then append the
 // triangle to every tile of the box, and lane 0 stores the box for the raster.
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws, const Frame* __restrict__ frames,
-                                                 int tws, int ths, uint32_t* __restrict__ cursor,
-                                                 const uint64_t* __restrict__ off, uint32_t* __restrict__ pool,
-                                                 uint32_t* __restrict__ overflow,
-                                                 const uint32_t* __restrict__ careful_cnt,
-                                                 const uint2* __restrict__ careful_in, uint4* __restrict__ careful_box,
-                                                 uint32_t careful_cap)
+__device__ __forceinline__ void careful_one(const Draw* __restrict__ draws, const Frame* __restrict__ frames, int tws,
+                                            int ths, uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
+                                            uint32_t* __restrict__ pool, uint32_t* __restrict__ overflow,
+                                            const uint2* __restrict__ careful_in, uint4* __restrict__ careful_box,
+                                            uint32_t w, int lane)
 {
-    const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = (int)__lane_id();
-    const uint32_t n = min(*careful_cnt, careful_cap);
-    if (w >= n) return;
     const uint2 in = careful_in[w];
     const Draw& dr = draws[in.x];
     const uint32_t frame = dr.frame;
@@ -386,6 +380,23 @@ __global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws,
         put_entry<COUNT>(gt, pos, CAREFUL_FLAG | w, off, pool, overflow, frame);
     }
 }
+
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws, const Frame* __restrict__ frames,
+                                                 int tws, int ths, uint32_t* __restrict__ cursor,
+                                                 const uint64_t* __restrict__ off, uint32_t* __restrict__ pool,
+                                                 uint32_t* __restrict__ overflow,
+                                                 const uint32_t* __restrict__ careful_cnt,
+                                                 const uint2* __restrict__ careful_in, uint4* __restrict__ careful_box,
+                                                 uint32_t careful_cap)
+{
+    const int lane = (int)__lane_id();
+    const uint32_t n = min(*careful_cnt, careful_cap);
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += n_waves)
+        careful_one<COUNT>(draws, frames, tws, ths, cursor, off, pool, overflow, careful_in, careful_box, w, lane);
+}
+
 
 // ---------------------------------------------------------------------------
 // k_tile_raster
@@ -764,7 +775,8 @@ int TiledPlan::launch_careful(const Frame* d_frames, const Draw* d_draws, bool c
 {
     if (careful_cap == 0) return SLOTH_OK;
     const int tws = log2u(TW), ths = log2u(TH);
-    dim3 grid(blocks(careful_cap, 4));   // 4 wavefronts per 256-thread block, one per careful triangle
+    // a fixed grid of wavefronts strides over the careful list (one triangle per wavefront at a time)
+    dim3 grid(std::min(blocks(careful_cap, 4), 1024u));
     if (count_only)
         hipLaunchKernelGGL(k_careful<true>, grid, dim3(256), 0, s, d_draws, d_frames, tws, ths,
                            d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),

@@ -163,9 +163,7 @@ def value_noise(p: np.ndarray, seed: int, freq: float = 4.0) -> np.ndarray:
 def displaced_icosphere(level: int, seed: int, amplitude: float = 0.05,
                         color_seed: int | None = None) -> Mesh:
     """Icosphere with radius 1 + amplitude * value_noise(seed); triangle areas
-    then vary, and displaced vertices can exceed the bounding-box maximum
-    along other directions, so the screen transform's scale does not bound
-    them (the row-wrap aliasing of rasterizer.rs:80 is exercised)."""
+    then vary from sub-pixel to tens of pixels at 1080p."""
     v, f = icosphere_indexed(level)
     r = 1.0 + amplitude * value_noise(v, seed)
     v = v * r[:, None]
