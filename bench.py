@@ -110,10 +110,13 @@ def main() -> int:
         t = prof[dominant] / 1e3
         achieved = alg_bytes / t / 1e9
         traffic = None
-        tf = os.path.join(ROOT, "profiles", f"traffic_{dominant}_sweep_1080p.json")
+        # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE summary of this kernel
+        tf = os.path.join(ROOT, "profiles", f"{dominant}_sweep_1080p.md")
         if os.path.exists(tf):
             with open(tf) as fh:
-                traffic = json.load(fh).get("hbm_bytes_per_launch")
+                for line in fh:
+                    if line.startswith("hbm_bytes_per_launch:"):
+                        traffic = int(line.split(":")[1])
         roof = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "alg_bytes_per_launch": alg_bytes, "kernel_ms": round(prof[dominant], 4),
