@@ -79,6 +79,18 @@ static void screen_transform(uint32_t W, uint32_t H, const float* bbox_max, uint
 
 static const float IDENTITY[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
 
+// Marks tables whose thresholds are non-decreasing with no NaN (the device
+// then counts thresholds below the shade instead of scanning).
+static void finish_ramp(Ramp& r)
+{
+    bool ok = true;
+    for (uint32_t i = 0; i < r.n; ++i) {
+        if (r.thr[i] != r.thr[i]) ok = false;
+        if (i + 1 < r.n && r.thr[i] > r.thr[i + 1]) ok = false;
+    }
+    r.sorted = ok ? 1 : 0;
+}
+
 static Ramp default_ramp()
 {
     static const float t[9] = {0.20f, 0.30f, 0.40f, 0.50f, 0.60f, 0.70f, 0.80f, 0.90f, 1.0f};
@@ -91,6 +103,7 @@ static Ramp default_ramp()
         r.glyph[i] = (uint8_t)g[i];
     }
     r.fallback = ' ';
+    finish_ramp(r);
     return r;
 }
 
@@ -108,6 +121,7 @@ static int to_ramp(const sloth_ramp* s, Ramp& r)
         r.glyph[i] = s->glyphs[i];
     }
     r.fallback = s->fallback;
+    finish_ramp(r);
     return SLOTH_OK;
 }
 
@@ -198,7 +212,7 @@ struct Job {
         draws.assign(din.size(), Draw{});
         uint64_t cells = 0, z = 0, rows = 0, items = 0;
         size_t k = 0;
-        bool all_even = true;
+        bool all_even = true, small_frames = true;
         for (uint32_t f = 0; f < F; ++f) {
             Frame& fr = frames[f];
             fr.W = fin[f].W;
@@ -229,6 +243,7 @@ struct Job {
             }
             fr.draw_end = (uint32_t)k;
             fr.n_tris = tri;
+            if (tri >= (1u << 30)) small_frames = false;
             uint64_t WH = (uint64_t)fr.W * fr.H;
             cells += WH + (fr.image ? fr.H : 0);
             cells = (cells + 1) & ~1ull;   // even offsets: slots store cell pairs as one 8-byte word
@@ -241,7 +256,8 @@ struct Job {
         n_rows = rows;
         n_items = items;
         // the tiled path keeps a draw's shader index in 8 bits of an LDS word
-        path = (all_even && ramps.size() <= 256 && !force_global()) ? PATH_TILED : PATH_GLOBAL;
+        // and flags in the two top bits of a 32-bit tile-list entry
+        path = (all_even && small_frames && ramps.size() <= 256 && !force_global()) ? PATH_TILED : PATH_GLOBAL;
         int rc;
         if ((rc = d_frames.reserve(sizeof(Frame) * std::max<size_t>(1, frames.size())))) return rc;
         if ((rc = d_draws.reserve(sizeof(Draw) * std::max<size_t>(1, draws.size())))) return rc;
@@ -261,6 +277,15 @@ struct Job {
     static bool force_global()
     {
         const char* e = getenv("SLOTH_FORCE_GLOBAL");
+        return e && e[0] == '1';
+    }
+
+    // SLOTH_PIPELINE=1 captures batch graphs as a two-stream pipeline of
+    // frame groups (bins of one group beside the raster of the previous one).
+    // Off by default: measured slower than one stream on the 1080p sweep.
+    static bool pipeline()
+    {
+        const char* e = getenv("SLOTH_PIPELINE");
         return e && e[0] == '1';
     }
 
@@ -356,14 +381,15 @@ struct Job {
         return fail(SLOTH_EHIP, "tile lists kept overflowing");
     }
 
+    // s2 / join (optional): second stream and events for the pipelined tiled launch
     int launch(uint32_t f_lo, uint32_t f_hi, hipStream_t s, int32_t* win, std::vector<hipEvent_t>* ev = nullptr,
-               bool win_raw = false)
+               bool win_raw = false, hipStream_t s2 = nullptr, const std::vector<hipEvent_t>* join = nullptr)
     {
         if (f_lo >= f_hi) return SLOTH_OK;
         if (path == PATH_TILED)
             return tiled.launch(frames, draws, d_frames.as<Frame>(), d_draws.as<Draw>(), d_ramps.as<Ramp>(), f_lo,
                                 f_hi, d_cells.as<uint32_t>(), d_z.as<float>(), win, win_raw, (uint32_t)ramps.size(), s,
-                                ev);
+                                ev, s2, join);
         return launch_global(f_lo, f_hi, s, win, win_raw, ev);
     }
 
@@ -446,6 +472,8 @@ struct sloth_batch {
     Job job;
     std::vector<float> U;       // per frame utransform
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;       // second branch of the captured pipeline
+    std::vector<hipEvent_t> join;        // its fork / join events
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t exec_generation = 0;   // tile plan generation the graph was captured with
@@ -457,6 +485,9 @@ struct sloth_batch {
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : join)
+            if (e) (void)hipEventDestroy(e);
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -781,10 +812,19 @@ int sloth_batch_create(const sloth_frame_desc* frames, uint32_t n_frames, const 
     sloth_batch* b = new (std::nothrow) sloth_batch();
     if (!b) return fail(SLOTH_ENOMEM, "host allocation failed");
     b->stream = make_stream();
-    if (!b->stream) {
+    b->stream2 = make_stream();
+    if (!b->stream || !b->stream2) {
         delete b;
         return fail(SLOTH_EHIP, "hipStreamCreate failed");
     }
+    b->join.assign(MAX_GROUPS + 1, nullptr);
+    for (hipEvent_t& e : b->join)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            e = nullptr;
+            delete b;
+            return fail(SLOTH_EHIP, "hipEventCreate failed");
+        }
     uint32_t ri = b->job.add_ramp(r);
     b->U.resize((size_t)n_frames * 16);
     for (uint32_t f = 0; f < n_frames; ++f) {
@@ -851,7 +891,9 @@ int sloth_batch_run(sloth_batch* b, void* stream)
     if (b->exec && b->exec_generation != b->job.tiled.generation) drop_graph(b);
     if (!b->exec) {
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        rc = b->job.launch(0, (uint32_t)b->job.frames.size(), b->stream, nullptr);
+        const bool pipe = Job::pipeline();
+        rc = b->job.launch(0, (uint32_t)b->job.frames.size(), b->stream, nullptr, nullptr, false,
+                           pipe ? b->stream2 : nullptr, pipe ? &b->join : nullptr);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (rc) {

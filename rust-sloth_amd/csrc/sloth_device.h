@@ -21,7 +21,8 @@ struct Ramp {
     float thr[MAX_RAMP];
     uint8_t glyph[MAX_RAMP];
     uint8_t fallback;
-    uint8_t pad[3];
+    uint8_t sorted;   // thresholds non-decreasing and none NaN (set by the host)
+    uint8_t pad[2];
 };
 
 // One draw_mesh call (rasterizer.rs:39-46) inside one frame.
@@ -155,6 +156,14 @@ __device__ __forceinline__ float shade_value(float nzu, float a, float w0, float
 
 __device__ __forceinline__ uint8_t ramp_glyph(const Ramp& r, float s)
 {
+    if (r.sorted) {
+        // Non-decreasing thresholds: {k : thr[k] < s} is a prefix [0, i), so
+        // i is the first index with s <= thr[i] (none when i == n); a NaN
+        // shade passes no threshold.  Counting avoids a divergent early exit.
+        uint32_t i = 0;
+        for (uint32_t k = 0; k < r.n; ++k) i += (r.thr[k] < s) ? 1u : 0u;
+        return (i == r.n || s != s) ? r.fallback : r.glyph[i];
+    }
     for (uint32_t i = 0; i < r.n; ++i)
         if (s <= r.thr[i]) return r.glyph[i];
     return r.fallback;
@@ -196,17 +205,28 @@ __device__ __forceinline__ void key_pixel(uint32_t id, uint32_t sub, uint32_t W,
 }
 
 // Classification of a triangle's candidate pixels (see setup_class).
-enum : uint32_t { CLS_EMPTY = 0, CLS_BOX = 1, CLS_SPANS = 2, CLS_FULL = 3 };
+enum : uint32_t { CLS_EMPTY = 0, CLS_BOX = 1, CLS_SPANS = 2, CLS_FULL = 3, CLS_TIGHT = 4 };
 
 // Conservative proof that the reference's FP32 edge test cannot pass outside
-// the box [floor(minx), ceil(maxx)] (CLS_BOX) or anywhere at all (back face,
-// CLS_EMPTY).  Uses |w_fp - w_exact| <= 4.0001 u (|A| |py-ay| + |B| |px-ax|)
-// per edge over the whole pixel domain, and that the exact edge functions sum
-// to the exact orient(v1,v2,v3).  Triangles the bound cannot decide, and those
-// with coordinates outside +-2^22, fall to CLS_SPANS (exact per-row spans found
-// from the monotonicity of each FP32 edge function along a row) or CLS_FULL
-// (every pixel of the domain).  u = 2^-24; the factor 8 covers the bound's own
-// rounding.
+// the box [floor(minx), ceil(maxx)] (CLS_BOX), outside the tight box
+// [ceil(minx - 1/32), floor(maxx + 1/32)] (CLS_TIGHT), or anywhere at all (back
+// face, CLS_EMPTY).  Uses |w_fp - w_exact| <= 4.0001 u (|A| |py-ay| + |B|
+// |px-ax|) per edge over the whole pixel domain, and that the exact edge
+// functions sum to the exact orient(v1,v2,v3) = o.
+//
+// Box argument: a pixel d to the left of minx (or right of maxx) has some
+// exact barycentric lambda_i <= -d / (2 (maxx - minx)) (at most two of the
+// three can be negative, and they carry the x offset), so its exact w_i <=
+// -o d / (2 width) and its FP32 w_i < 0 once o d / (2 width) > bound.  With
+// bound * width < lo / 2 that holds for d >= 1 (CLS_BOX); with bound * width
+// < lo / 128 for d >= 1/64 (CLS_TIGHT), and every integer x >= minx - 1/64 is
+// >= fl(minx - 1/32) (the subtraction rounds by at most 1/64 when ulp(minx) <=
+// 1/32, and when ulp(minx) >= 1/16 minx sits on a 1/16 grid, so x >= minx).
+//
+// Triangles the bound cannot decide, and those with coordinates outside
+// +-2^22, fall to CLS_SPANS (exact per-row spans found from the monotonicity of
+// each FP32 edge function along a row) or CLS_FULL (every pixel of the
+// domain).  u = 2^-24; the factor 8 covers the bound's own rounding.
 __device__ __forceinline__ uint32_t setup_class(const Tri& t)
 {
     if (t.x0 >= t.x1d || t.y0 >= t.y1d) return CLS_EMPTY;
@@ -227,8 +247,25 @@ __device__ __forceinline__ uint32_t setup_class(const Tri& t)
     float eo = K * (fabsf(t.A2) * fabsf(ey) + fabsf(t.B2) * fabsf(ex));
     if (o + eo < -bound) return CLS_EMPTY;          // back face, provably uncovered
     float lo = o - eo;
-    if (lo > 0.0f && bound * (t.maxx - t.minx) < 0.5f * lo) return CLS_BOX;
+    if (lo > 0.0f) {
+        const float bw = bound * (t.maxx - t.minx);
+        if (bw < 0.0078125f * lo) return CLS_TIGHT;
+        if (bw < 0.5f * lo) return CLS_BOX;
+    }
     return CLS_SPANS;
+}
+
+// x range of the candidate box of a CLS_BOX / CLS_TIGHT triangle, clipped to
+// its domain (empty when bx0 > bx1).  |minx|, |maxx| < 2^22 for both classes.
+__device__ __forceinline__ void box_x(const Tri& t, bool tight, int& bx0, int& bx1)
+{
+    if (tight) {
+        bx0 = max(t.x0, (int)ceilf(t.minx - 0.03125f));
+        bx1 = min(t.x1d - 1, (int)floorf(t.maxx + 0.03125f));
+    } else {
+        bx0 = max(t.x0, (int)floorf(t.minx));
+        bx1 = min(t.x1d - 1, (int)ceilf(t.maxx));
+    }
 }
 
 // Exact covered x-range on row y for CLS_SPANS triangles: each edge function
@@ -284,16 +321,15 @@ __device__ __forceinline__ bool row_span(const Tri& t, int y, int& lo, int& hi)
 
 // Candidate box [bx0,bx1] x [by0,by1] (logical pixels, inclusive) that holds
 // every pixel of the triangle's domain the FP32 edge test can pass; false when
-// there is none.  CLS_BOX uses the proof in setup_class, CLS_SPANS the union of
+// there is none.  CLS_BOX / CLS_TIGHT use the proof in setup_class, CLS_SPANS the union of
 // the exact row spans, CLS_FULL the whole domain.
 __device__ __forceinline__ bool tri_box(const Tri& t, uint32_t cls, int& bx0, int& bx1, int& by0, int& by1)
 {
     if (cls == CLS_EMPTY) return false;
     by0 = t.y0;
     by1 = t.y1d - 1;
-    if (cls == CLS_BOX) {
-        bx0 = max(t.x0, (int)floorf(t.minx));
-        bx1 = min(t.x1d - 1, (int)ceilf(t.maxx));
+    if (cls == CLS_BOX || cls == CLS_TIGHT) {
+        box_x(t, cls == CLS_TIGHT, bx0, bx1);
         return bx0 <= bx1;
     }
     if (cls == CLS_FULL) {

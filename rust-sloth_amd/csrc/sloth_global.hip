@@ -74,9 +74,9 @@ __global__ void __launch_bounds__(256) k_raster_global(const Draw* __restrict__ 
 
     uint32_t cls = setup_class(t);
     if (cls == CLS_EMPTY) return;
-    if (cls == CLS_BOX) {
-        int xa = max(t.x0, (int)floorf(t.minx));
-        int xb = min(t.x1d - 1, (int)ceilf(t.maxx));
+    if (cls == CLS_BOX || cls == CLS_TIGHT) {
+        int xa, xb;
+        box_x(t, cls == CLS_TIGHT, xa, xb);
         for (int y = t.y0; y < t.y1d; ++y)
             for (int x = xa; x <= xb; ++x) plot(fk, t, W, tri, x, y);
     } else if (cls == CLS_SPANS) {

@@ -12,12 +12,13 @@
 //                  row), and the triangle index appended to every tile its box
 //                  reaches, with one atomicAdd per distinct tile per wavefront
 //                  (ballot + prefix popcount)
-//   k_tile_raster  one 256-thread workgroup per tile: each list entry's transformed
-//                  triangle goes to an LDS row (found by a hash of its index),
-//                  its fragments do a 64-bit LDS atomicMin of the packed keys,
-//                  then every slot re-evaluates its winner from the LDS row
-//                  and stores its two cells and depths (coalesced).  Lists
-//                  longer than the LDS rows take two walks instead.
+//   k_tile_raster  one 256-thread workgroup per tile, grid (tile x, tile y,
+//                  frame): each list entry's transformed triangle goes to an
+//                  LDS row (found by a hash of its index), its fragments do a
+//                  64-bit LDS atomicMin of the packed keys, then every slot
+//                  re-evaluates its winner from the LDS row and stores its two
+//                  cells and depths (coalesced).  Lists longer than the 256
+//                  LDS rows run in chunks, staging winners after each chunk.
 //
 // Tile lists live in one pool at fixed per-tile offsets, sized from an exact
 // counting run of k_bin plus headroom.  If a later run would exceed a tile's
@@ -32,6 +33,9 @@
 #include "sloth_host.h"
 
 namespace sloth {
+
+constexpr uint32_t MAX_GROUPS = 8;        // frame groups of a pipelined launch
+constexpr uint32_t PIPE_MIN_FRAMES = 16;  // frames per group at least
 
 struct TiledPlan {
     uint32_t TW = 64, TH = 16;       // tile size in cells (TW even)
@@ -50,7 +54,7 @@ struct TiledPlan {
     // state.  Pool offsets become invalid until size() runs.
     int plan(std::vector<Frame>& frames, uint64_t rows);
 
-    // Tile size in cells: TW a power of two in 8..128, TH in 4..64.
+    // Tile size in cells: TW a power of two in 8..128, TH in 4..64, TW * TH <= 4096.
     int set_tile(uint32_t tw, uint32_t th);
     size_t raster_lds() const;
 
@@ -62,10 +66,14 @@ struct TiledPlan {
     int upload(hipStream_t s) { (void)s; return SLOTH_OK; }
 
     // clear + bin + careful + raster over frames [f_lo, f_hi).  ev (optional)
-    // gets 5 events recorded around the 4 stages.
+    // gets 5 events recorded around the 4 stages.  With a second stream s2
+    // and join events (and no ev), ranges of at least 2 * PIPE_MIN_FRAMES
+    // frames run as a pipeline of frame groups: the bins of one group overlap
+    // the raster of the previous one (join needs groups + 1 events).
     int launch(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
                const Draw* d_draws, const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells, float* z,
-               int32_t* win, bool win_raw, uint32_t n_ramps, hipStream_t s, std::vector<hipEvent_t>* ev);
+               int32_t* win, bool win_raw, uint32_t n_ramps, hipStream_t s, std::vector<hipEvent_t>* ev,
+               hipStream_t s2 = nullptr, const std::vector<hipEvent_t>* join = nullptr);
 
     // Reads the overflow flags of frames [0, nf) (synchronises the stream).
     int overflowed(uint32_t nf, hipStream_t s, bool& ovf);
@@ -76,7 +84,10 @@ struct TiledPlan {
     int launch_clear(const std::vector<Frame>& frames, uint32_t f_lo, uint32_t f_hi, hipStream_t s);
     int launch_bin(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
                    const Draw* d_draws, uint32_t f_lo, uint32_t f_hi, bool count_only, hipStream_t s);
-    int launch_careful(const Frame* d_frames, const Draw* d_draws, bool count_only, hipStream_t s);
+    int launch_careful(const Frame* d_frames, const Draw* d_draws, bool count_only, uint32_t group, hipStream_t s);
+    int launch_raster(const std::vector<Frame>& frames, const Frame* d_frames, const Draw* d_draws,
+                      const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells, float* z, int32_t* win,
+                      bool win_raw, uint32_t n_ramps, hipStream_t s);
 };
 
 }  // namespace sloth

@@ -22,17 +22,20 @@ __global__ void k_tile_clear(uint32_t* __restrict__ cursor, uint64_t n_tiles, in
     for (uint64_t i = i0; i < n_tiles; i += stride) cursor[i] = 0;
     for (uint64_t i = i0; i < n_rows; i += stride) nl[i] = -1;
     for (uint64_t i = i0; i < n_frames; i += stride) overflow[i] = 0;
-    if (i0 == 0) *careful_cnt = 0;
+    if (i0 == 0) careful_cnt[0] = careful_cnt[1] = 0;   // list length, first group's start
 }
 
 // ---------------------------------------------------------------------------
 // tile geometry shared by the binning kernels
 // ---------------------------------------------------------------------------
 
-// Pool entries: the frame triangle index of a CLS_BOX triangle (its box is
-// recomputed from floor/ceil of its x extent), or CAREFUL_FLAG | row of the
-// careful-box table for triangles whose box k_careful computed.
+// Pool entries: the frame triangle index of a CLS_TIGHT triangle, LOOSE_FLAG |
+// the index of a CLS_BOX one (the raster recomputes either box from the
+// triangle's x extent), or CAREFUL_FLAG | row of the careful-box table for
+// triangles whose box k_careful computed.  Frames on this path hold fewer than
+// 2^30 triangles.
 constexpr uint32_t CAREFUL_FLAG = 0x80000000u;
+constexpr uint32_t LOOSE_FLAG = 0x40000000u;
 constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
 
 // Tile rectangles (inclusive, in tile units) that a candidate box reaches:
@@ -199,6 +202,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 
     int y0 = 0, y1d = 0;
     bool careful = false;
+    uint32_t entry = tri;
     TileRects r;
     empty_rects(r);
     if (valid) {
@@ -207,8 +211,10 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         y0 = t.y0;
         y1d = t.y1d;
         const uint32_t cls = setup_class(t);
-        if (cls == CLS_BOX) {
-            const int bx0 = max(t.x0, (int)floorf(t.minx)), bx1 = min(t.x1d - 1, (int)ceilf(t.maxx));
+        if (cls == CLS_BOX || cls == CLS_TIGHT) {
+            int bx0, bx1;
+            box_x(t, cls == CLS_TIGHT, bx0, bx1);
+            if (cls == CLS_BOX) entry |= LOOSE_FLAG;
             if (bx0 <= bx1) tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);
         } else if (cls != CLS_EMPTY) {
             careful = true;
@@ -290,11 +296,11 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 
     if (inwin) {
         const uint64_t g0 = tile_begin + (uint64_t)r.dy0 * tiles_x + (uint64_t)r.dx0;
-        put_entry<COUNT>(g0, w_base[c] + lp0, tri, off, pool, overflow, frame);
-        if (wide) put_entry<COUNT>(g0 + 1, w_base[c + 1] + lp1, tri, off, pool, overflow, frame);
-        if (tall) put_entry<COUNT>(g0 + tiles_x, w_base[c + BIN_WIN] + lp2, tri, off, pool, overflow, frame);
+        put_entry<COUNT>(g0, w_base[c] + lp0, entry, off, pool, overflow, frame);
+        if (wide) put_entry<COUNT>(g0 + 1, w_base[c + 1] + lp1, entry, off, pool, overflow, frame);
+        if (tall) put_entry<COUNT>(g0 + tiles_x, w_base[c + BIN_WIN] + lp2, entry, off, pool, overflow, frame);
         if (wide && tall)
-            put_entry<COUNT>(g0 + tiles_x + 1, w_base[c + BIN_WIN + 1] + lp3, tri, off, pool, overflow, frame);
+            put_entry<COUNT>(g0 + tiles_x + 1, w_base[c + BIN_WIN + 1] + lp3, entry, off, pool, overflow, frame);
     } else if (has_direct || has_wrap) {
         // large or wrapping boxes: one global atomic per tile
         TileIter it;
@@ -302,7 +308,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         while (it.phase < 2) {
             const uint64_t gt = tile_begin + (uint64_t)it.ty * tiles_x + (uint64_t)it.tx;
             const uint32_t pos = atomicAdd(&cursor[gt], 1u);
-            put_entry<COUNT>(gt, pos, tri, off, pool, overflow, frame);
+            put_entry<COUNT>(gt, pos, entry, off, pool, overflow, frame);
             iter_next(r, it);
         }
     }
@@ -381,19 +387,24 @@ __device__ __forceinline__ void careful_one(const Draw* __restrict__ draws, cons
     }
 }
 
+// careful_cnt[0] is the list length; careful_cnt[1 + g] the index where group
+// g's triangles start (the bins of group g ran after those of group g - 1).
+// Group g takes [careful_cnt[1 + g], length) and records the length as group
+// g + 1's start.
 template <bool COUNT>
 __global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws, const Frame* __restrict__ frames,
                                                  int tws, int ths, uint32_t* __restrict__ cursor,
                                                  const uint64_t* __restrict__ off, uint32_t* __restrict__ pool,
-                                                 uint32_t* __restrict__ overflow,
-                                                 const uint32_t* __restrict__ careful_cnt,
-                                                 const uint2* __restrict__ careful_in, uint4* __restrict__ careful_box,
-                                                 uint32_t careful_cap)
+                                                 uint32_t* __restrict__ overflow, uint32_t* __restrict__ careful_cnt,
+                                                 uint32_t group, const uint2* __restrict__ careful_in,
+                                                 uint4* __restrict__ careful_box, uint32_t careful_cap)
 {
     const int lane = (int)__lane_id();
-    const uint32_t n = min(*careful_cnt, careful_cap);
+    const uint32_t n = min(careful_cnt[0], careful_cap);
+    const uint32_t first = careful_cnt[1 + group];
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += n_waves)
+    if (blockIdx.x == 0 && threadIdx.x == 0) careful_cnt[2 + group] = n;
+    for (uint32_t w = first + ((blockIdx.x * blockDim.x + threadIdx.x) >> 6); w < n; w += n_waves)
         careful_one<COUNT>(draws, frames, tws, ths, cursor, off, pool, overflow, careful_in, careful_box, w, lane);
 }
 
@@ -401,10 +412,11 @@ __global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws,
 // ---------------------------------------------------------------------------
 // k_tile_raster
 // ---------------------------------------------------------------------------
-constexpr uint32_t CACHE_CAP = 256;   // list length up to which the fast path runs
-constexpr uint32_t HASH_CAP = 512;    // open-addressing table: triangle -> cache row
+constexpr uint32_t CHUNK = RASTER_BLOCK;   // list entries whose triangles sit in LDS at once (one per thread)
+constexpr uint32_t HASH_CAP = 512;         // open-addressing table: triangle -> cache row
 constexpr uint32_t CACHE_WORDS = 12;
-constexpr uint32_t LDS_RAMPS = 4;     // shaders staged in LDS per tile
+constexpr uint32_t LDS_RAMPS = 4;          // shaders staged in LDS per tile
+constexpr uint32_t MAX_SLOTS = 2048;       // (TW / 2) * TH: bounds the workgroup's LDS
 
 __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws, const Frame& fr, uint32_t tri)
 {
@@ -443,12 +455,14 @@ __device__ __forceinline__ void walk_tile(const TileGeom& g, int bx0, int bx1, i
         for (int x = xa; x <= xb; ++x) f(x, y, ((y + 1 - g.tr0) << g.sws) + (x - wx0), 0u);
 }
 
-// Triangle index of a pool entry and, for careful entries, its box.
+// Triangle index of a pool entry; the candidate box of a careful entry comes
+// from its table row (given = true), the others' from box_x.
 __device__ __forceinline__ uint32_t entry_tri(uint32_t e, const uint4* __restrict__ careful_box, int& bx0, int& bx1,
-                                              int& by0, int& by1, bool& given)
+                                              int& by0, int& by1, bool& given, bool& tight)
 {
     given = (e & CAREFUL_FLAG) != 0;
-    if (!given) return e;
+    tight = (e & LOOSE_FLAG) == 0;
+    if (!given) return e & ~LOOSE_FLAG;
     const uint4 b = careful_box[e & ~CAREFUL_FLAG];
     bx0 = (int)b.y;
     bx1 = (int)b.z;
@@ -457,43 +471,27 @@ __device__ __forceinline__ uint32_t entry_tri(uint32_t e, const uint4* __restric
     return b.x;
 }
 
-// Candidate box of a CLS_BOX entry (the proof in setup_class holds for it).
-__device__ __forceinline__ bool box_of(const Tri& t, int& bx0, int& bx1, int& by0, int& by1)
+// One workgroup per tile; grid (tile x, tile y, frame - f_lo).  Blocks past a
+// smaller frame's tile grid return at once.
+//
+// The tile's list is taken CHUNK entries at a time: each entry's transformed
+// triangle goes to an LDS row found through a hash of its triangle index, and
+// its fragments take a 64-bit LDS atomicMin of the packed (depth, triangle,
+// sub) keys.  With one chunk, every slot then re-evaluates its winner from
+// the LDS row.  With several, after each chunk the slots whose key belongs to
+// one of the chunk's triangles stage their cell and depth (the last staging of
+// a slot is its final winner's, since a later winner comes from a later chunk).
+// Tile (tx, ty) of frame fr, whose list is pool[o, o + n).
+__device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, const Ramp* __restrict__ ramps,
+                                            const Ramp* s_ramp, const Frame& fr, uint32_t tx, uint32_t ty, uint64_t o,
+                                            uint32_t n, int tws, int ths, const uint32_t* __restrict__ pool,
+                                            const int32_t* __restrict__ nl, const uint4* __restrict__ careful_box,
+                                            uint32_t* __restrict__ cells, float* __restrict__ zbuf,
+                                            int32_t* __restrict__ win, uint32_t win_raw, uint64_t* smem)
 {
-    bx0 = max(t.x0, (int)floorf(t.minx));
-    bx1 = min(t.x1d - 1, (int)ceilf(t.maxx));
-    by0 = t.y0;
-    by1 = t.y1d - 1;
-    return bx0 <= bx1 && by0 <= by1;
-}
-
-__global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
-    const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
-    uint32_t f_hi, uint64_t tile_base, int tws, int ths, const uint32_t* __restrict__ cursor,
-    const uint64_t* __restrict__ off, const uint32_t* __restrict__ pool, const int32_t* __restrict__ nl,
-    const uint4* __restrict__ careful_box, uint32_t* __restrict__ cells, float* __restrict__ zbuf,
-    int32_t* __restrict__ win, uint32_t win_raw, uint32_t n_ramps)
-{
-    extern __shared__ uint64_t smem[];
-    // the job's first LDS_RAMPS shaders, so that a glyph lookup reads LDS
-    __shared__ Ramp s_ramp[LDS_RAMPS];
-    for (uint32_t i = threadIdx.x; i < min(n_ramps, LDS_RAMPS) * (uint32_t)sizeof(Ramp) / 4; i += blockDim.x)
-        ((uint32_t*)s_ramp)[i] = ((const uint32_t*)ramps)[i];
-    auto ramp_of = [&](uint32_t r) -> const Ramp& { return r < LDS_RAMPS ? s_ramp[r] : ramps[r]; };
+    const uint32_t W = fr.W, H = fr.H;
     const int sws = tws - 1;
     const uint32_t S = 1u << (sws + ths);
-    uint64_t* keys = smem;
-
-    const uint64_t gt = tile_base + blockIdx.x;
-    uint32_t lo = f_lo, hi = f_hi - 1;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (frames[mid].tile_begin <= gt) lo = mid; else hi = mid - 1;
-    }
-    const Frame& fr = frames[lo];
-    const uint32_t W = fr.W, H = fr.H;
-    const uint32_t local = (uint32_t)(gt - fr.tile_begin);
-    const uint32_t ty = local / fr.tiles_x, tx = local - ty * fr.tiles_x;
     TileGeom g;
     g.tc0 = (int)(tx << tws);
     g.tc1 = min((int)W, g.tc0 + (1 << tws));
@@ -502,31 +500,14 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
     g.sws = sws;
     g.W = (int)W;
 
-    const uint64_t o = off[gt];
-    const uint32_t n = min(cursor[gt], (uint32_t)(off[gt + 1] - o));
     const uint32_t* list = pool + o;
-    const bool one_draw = fr.draw_end - fr.draw_begin == 1;
-    auto draw_for = [&](uint32_t T) -> const Draw& {
-        return one_draw ? draws[fr.draw_begin] : draw_of_t(draws, fr, T);
-    };
-    auto test = [&](const Tri& t, uint32_t T, int bx0, int bx1, int by0, int by1) {
-        walk_tile(g, bx0, bx1, by0, by1, [&](int x, int y, int slot, uint32_t sub) {
-            float w0, w1, w2;
-            edges(t, (float)x, (float)y, w0, w1, w2);
-            if (!inside(w0, w1, w2)) return;
-            float z = depth(t, w1, w2);
-            if (!(z < F32_MAX)) return;
-            atomicMin((unsigned long long*)&keys[slot], (unsigned long long)make_key(z, T, sub));
-        });
-    };
 
     const int32_t* fnl = nl + fr.nl_off;
     uint32_t* fc = cells + fr.cell_off;
     float* fz = zbuf + fr.z_off;
     // Stores the cells and depths of slot s from its winning cell (or blank).
     auto store = [&](uint32_t s, uint64_t k, uint32_t cell, float z) {
-        const int sr = (int)(s >> sws), sc = (int)(s & ((1u << sws) - 1));
-        const int row = g.tr0 + sr, col = g.tc0 + 2 * sc;
+        const int row = g.tr0 + (int)(s >> sws), col = g.tc0 + 2 * (int)(s & ((1u << sws) - 1));
         uint32_t c0 = CELL_BLANK, c1 = CELL_BLANK;
         float z0 = F32_MAX;
         int32_t wt = -1;
@@ -561,123 +542,147 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
         // newline cells of column 1
         for (uint32_t s = threadIdx.x; s < S; s += blockDim.x)
             if (slot_valid(s)) store(s, KEY_EMPTY, CELL_BLANK, F32_MAX);
-    } else if (n <= CACHE_CAP) {
-        // Fast path: one walk.  Each entry's transformed triangle goes to an
-        // LDS row found through a hash of its triangle index; the winner of a
-        // slot is re-evaluated from that row.
-        uint32_t* htag = (uint32_t*)(keys + S);
+    } else {
+        auto ramp_of = [&](uint32_t r) -> const Ramp& { return r < LDS_RAMPS ? s_ramp[r] : ramps[r]; };
+        uint64_t* keys = smem;
+        uint32_t* st_cell = (uint32_t*)(keys + S);
+        float* st_z = (float*)(st_cell + S);
+        uint32_t* htag = (uint32_t*)(st_z + S);
         uint32_t* hrow = htag + HASH_CAP;
         float* cache = (float*)(hrow + HASH_CAP);
-        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
-        for (uint32_t h = threadIdx.x; h < HASH_CAP; h += blockDim.x) htag[h] = EMPTY32;
-        __syncthreads();
-        for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
-            int bx0, bx1, by0, by1;
-            bool given;
-            const uint32_t T = entry_tri(list[p], careful_box, bx0, bx1, by0, by1, given);
-            const Draw& dr = draw_for(T);
-            const uint32_t li = T - dr.tri_begin;
-            Tri t;
-            setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
-            const float na = unit_normal_z(t) * t.a;   // rasterizer.rs:76 multiplies these first
-            const uint8_t* c = dr.rgb + (size_t)li * 3;
-            const uint32_t cw = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24) | dr.ramp;
-            float4* row4 = (float4*)(cache + p * CACHE_WORDS);
-            row4[0] = make_float4(t.x1, t.y1, t.x2, t.y2);
-            row4[1] = make_float4(t.x3, t.y3, t.z1, t.z2 - t.z1);
-            row4[2] = make_float4(t.z3 - t.z1, t.a, na, __uint_as_float(cw));
-            uint32_t h = hash_tri(T);
+        const bool one_draw = fr.draw_end - fr.draw_begin == 1;
+        const bool multi = n > CHUNK;
+
+        // Winner of slot s with key k, re-evaluated from its cache row; false
+        // when the key's triangle is not in the current chunk.
+        auto winner = [&](uint32_t s, uint64_t k, uint32_t& cell, float& z) -> bool {
+            const uint32_t T = (uint32_t)k >> 1, sub = (uint32_t)k & 1u;
+            uint32_t h = hash_tri(T), tag;
             for (uint32_t probe = 0; probe < HASH_CAP; ++probe) {
-                if (atomicCAS(&htag[h], EMPTY32, T) == EMPTY32) {
-                    hrow[h] = p;
-                    break;
-                }
+                tag = htag[h];
+                if (tag == T || tag == EMPTY32) break;
                 h = (h + 1) & (HASH_CAP - 1);
             }
-            if (given ? (bx0 <= bx1 && by0 <= by1) : box_of(t, bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
+            if (tag != T) return false;
+            const float4* row4 = (const float4*)(cache + hrow[h] * CACHE_WORDS);
+            const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
+            const int row = g.tr0 + (int)(s >> sws), col = g.tc0 + 2 * (int)(s & ((1u << sws) - 1));
+            const float px = (float)(sub ? (col >> 1) : ((col + g.W) >> 1));
+            const float py = (float)(sub ? row : row - 1);
+            // the same expressions as edges(), depth() and shade_value()
+            const float x1 = q0.x, y1 = q0.y, x2 = q0.z, y2 = q0.w, x3 = q1.x, y3 = q1.y;
+            const float w0 = (x3 - x2) * (py - y2) - (y3 - y2) * (px - x2);
+            const float w1 = (x1 - x3) * (py - y3) - (y1 - y3) * (px - x3);
+            const float w2 = (x2 - x1) * (py - y1) - (y2 - y1) * (px - x1);
+            z = q1.z + q2.y * (w1 * q1.w + w2 * q2.x);
+            const float sh = q2.z * (w0 + w1 + w2);
+            const uint32_t cw = __float_as_uint(q2.w);
+            cell = (uint32_t)ramp_glyph(ramp_of(cw & 0xFFu), sh) | (cw & 0xFFFFFF00u);
+            return true;
+        };
+
+        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
+        for (uint32_t base = 0; base < n; base += CHUNK) {
+            if (base) __syncthreads();   // the previous chunk's staging no longer reads its rows
+            for (uint32_t h = threadIdx.x; h < HASH_CAP; h += blockDim.x) htag[h] = EMPTY32;
+            __syncthreads();
+            const uint32_t p = threadIdx.x;
+            if (base + p < n) {
+                int bx0, bx1, by0, by1;
+                bool given, tight;
+                const uint32_t T = entry_tri(list[base + p], careful_box, bx0, bx1, by0, by1, given, tight);
+                const Draw& dr = one_draw ? draws[fr.draw_begin] : draw_of_t(draws, fr, T);
+                const uint32_t li = T - dr.tri_begin;
+                Tri t;
+                setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
+                const float na = unit_normal_z(t) * t.a;   // rasterizer.rs:76 multiplies these first
+                const uint8_t* c = dr.rgb + (size_t)li * 3;
+                const uint32_t cw = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24) | dr.ramp;
+                float4* row4 = (float4*)(cache + p * CACHE_WORDS);
+                row4[0] = make_float4(t.x1, t.y1, t.x2, t.y2);
+                row4[1] = make_float4(t.x3, t.y3, t.z1, t.z2 - t.z1);
+                row4[2] = make_float4(t.z3 - t.z1, t.a, na, __uint_as_float(cw));
+                uint32_t h = hash_tri(T);
+                for (uint32_t probe = 0; probe < HASH_CAP; ++probe) {
+                    if (atomicCAS(&htag[h], EMPTY32, T) == EMPTY32) {
+                        hrow[h] = p;
+                        break;
+                    }
+                    h = (h + 1) & (HASH_CAP - 1);
+                }
+                if (!given) {
+                    box_x(t, tight, bx0, bx1);
+                    by0 = t.y0;
+                    by1 = t.y1d - 1;
+                }
+                if (bx0 <= bx1 && by0 <= by1)
+                    walk_tile(g, bx0, bx1, by0, by1, [&](int x, int y, int slot, uint32_t sub) {
+                        float w0, w1, w2;
+                        edges(t, (float)x, (float)y, w0, w1, w2);
+                        const float z = depth(t, w1, w2);
+                        if (inside(w0, w1, w2) && z < F32_MAX)
+                            atomicMin((unsigned long long*)&keys[slot], (unsigned long long)make_key(z, T, sub));
+                    });
+            }
+            __syncthreads();
+            if (multi)
+                for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+                    const uint64_t k = keys[s];
+                    uint32_t cell;
+                    float z;
+                    if (k != KEY_EMPTY && winner(s, k, cell, z)) {
+                        st_cell[s] = cell;
+                        st_z[s] = z;
+                    }
+                }
         }
-        __syncthreads();
+        // the same thread staged slot s, so no barrier is needed before reading st_*
         for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
             if (!slot_valid(s)) continue;
             const uint64_t k = keys[s];
             uint32_t cell = CELL_BLANK;
             float z = F32_MAX;
             if (k != KEY_EMPTY) {
-                const uint32_t T = (uint32_t)k >> 1, sub = (uint32_t)k & 1u;
-                uint32_t h = hash_tri(T);
-                for (uint32_t probe = 0; probe < HASH_CAP && htag[h] != T; ++probe) h = (h + 1) & (HASH_CAP - 1);
-                const float4* row4 = (const float4*)(cache + hrow[h] * CACHE_WORDS);
-                const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
-                const int row = g.tr0 + (int)(s >> sws), col = g.tc0 + 2 * (int)(s & ((1u << sws) - 1));
-                const float px = (float)(sub ? (col >> 1) : ((col + g.W) >> 1));
-                const float py = (float)(sub ? row : row - 1);
-                // the same expressions as edges(), depth() and shade_value()
-                const float x1 = q0.x, y1 = q0.y, x2 = q0.z, y2 = q0.w, x3 = q1.x, y3 = q1.y;
-                const float w0 = (x3 - x2) * (py - y2) - (y3 - y2) * (px - x2);
-                const float w1 = (x1 - x3) * (py - y3) - (y1 - y3) * (px - x3);
-                const float w2 = (x2 - x1) * (py - y1) - (y2 - y1) * (px - x1);
-                z = q1.z + q2.y * (w1 * q1.w + w2 * q2.x);
-                const float sh = q2.z * (w0 + w1 + w2);
-                const uint32_t cw = __float_as_uint(q2.w);
-                cell = (uint32_t)ramp_glyph(ramp_of(cw & 0xFFu), sh) | (cw & 0xFFFFFF00u);
+                if (multi) {
+                    cell = st_cell[s];
+                    z = st_z[s];
+                } else {
+                    winner(s, k, cell, z);
+                }
             }
             store(s, k, cell, z);
         }
-    } else {
-        // Long lists: two walks; the second stages the winner's cell and depth.
-        uint32_t* st_cell = (uint32_t*)(keys + S);
-        float* st_z = (float*)(st_cell + S);
-        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
-        __syncthreads();
-        for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
-            int bx0, bx1, by0, by1;
-            bool given;
-            const uint32_t T = entry_tri(list[p], careful_box, bx0, bx1, by0, by1, given);
-            const Draw& dr = draw_for(T);
-            Tri t;
-            setup_tri(dr.M, dr.xyz + (size_t)(T - dr.tri_begin) * 9, W, H, t);
-            if (given ? (bx0 <= bx1 && by0 <= by1) : box_of(t, bx0, bx1, by0, by1)) test(t, T, bx0, bx1, by0, by1);
-        }
-        __syncthreads();
-        for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
-            int bx0, bx1, by0, by1;
-            bool given;
-            const uint32_t T = entry_tri(list[p], careful_box, bx0, bx1, by0, by1, given);
-            const Draw& dr = draw_for(T);
-            const uint32_t li = T - dr.tri_begin;
-            Tri t;
-            setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
-            if (!(given ? (bx0 <= bx1 && by0 <= by1) : box_of(t, bx0, bx1, by0, by1))) continue;
-            bool have_nz = false;
-            float na = 0.0f;
-            uint32_t color = 0;
-            walk_tile(g, bx0, bx1, by0, by1, [&](int x, int y, int slot, uint32_t sub) {
-                float w0, w1, w2;
-                edges(t, (float)x, (float)y, w0, w1, w2);
-                if (!inside(w0, w1, w2)) return;
-                float z = depth(t, w1, w2);
-                if (!(z < F32_MAX)) return;
-                if (keys[slot] != make_key(z, T, sub)) return;
-                if (!have_nz) {
-                    na = unit_normal_z(t) * t.a;
-                    const uint8_t* c = dr.rgb + (size_t)li * 3;
-                    color = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24);
-                    have_nz = true;
-                }
-                st_cell[slot] = (uint32_t)ramp_glyph(ramp_of(dr.ramp), na * (w0 + w1 + w2)) | color;
-                st_z[slot] = z;
-            });
-        }
-        __syncthreads();
-        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
-            if (!slot_valid(s)) continue;
-            const uint64_t k = keys[s];
-            store(s, k, k != KEY_EMPTY ? st_cell[s] : CELL_BLANK, k != KEY_EMPTY ? st_z[s] : F32_MAX);
-        }
     }
     // image mode: the H cells after the W*H grid are never written (context.rs:38-40)
-    if (fr.image && local == 0)
+    if (fr.image && tx == 0 && ty == 0)
         for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) fc[(uint64_t)W * H + i] = CELL_BLANK;
+}
+
+// One workgroup per tile; grid (tile x, tile y, frame - f_lo).  Blocks past a
+// smaller frame's tile grid return at once.
+__global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
+    const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
+    int tws, int ths, const uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ pool, const int32_t* __restrict__ nl, const uint4* __restrict__ careful_box,
+    uint32_t* __restrict__ cells, float* __restrict__ zbuf, int32_t* __restrict__ win, uint32_t win_raw,
+    uint32_t n_ramps)
+{
+    extern __shared__ uint64_t smem[];
+    // the job's first LDS_RAMPS shaders, so that a glyph lookup reads LDS
+    // (read only after the tile's first barrier)
+    __shared__ Ramp s_ramp[LDS_RAMPS];
+
+    const Frame& fr = frames[f_lo + blockIdx.z];
+    const uint32_t tx = blockIdx.x, ty = blockIdx.y;
+    if (tx >= fr.tiles_x || ty >= fr.tiles_y) return;
+    const uint64_t gt = fr.tile_begin + (uint64_t)ty * fr.tiles_x + tx;
+    const uint64_t o = off[gt];
+    const uint32_t n = min(cursor[gt], (uint32_t)(off[gt + 1] - o));
+    if (n > 0)
+        for (uint32_t i = threadIdx.x; i < min(n_ramps, LDS_RAMPS) * (uint32_t)sizeof(Ramp) / 4; i += blockDim.x)
+            ((uint32_t*)s_ramp)[i] = ((const uint32_t*)ramps)[i];
+    raster_tile(draws, ramps, s_ramp, fr, tx, ty, o, n, tws, ths, pool, nl, careful_box, cells, zbuf, win, win_raw,
+                smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -694,8 +699,10 @@ static int log2u(uint32_t v)
 
 int TiledPlan::set_tile(uint32_t tw, uint32_t th)
 {
-    if (tw < 8 || tw > 128 || th < 4 || th > 64 || (tw & (tw - 1)) || (th & (th - 1)))
-        return fail(SLOTH_EINVAL, "tile size %ux%u: width must be a power of two in 8..128, height in 4..64", tw, th);
+    if (tw < 8 || tw > 128 || th < 4 || th > 64 || (tw & (tw - 1)) || (th & (th - 1)) || (tw / 2) * th > MAX_SLOTS)
+        return fail(SLOTH_EINVAL,
+                    "tile size %ux%u: width must be a power of two in 8..128, height in 4..64, width * height <= %u",
+                    tw, th, 2 * MAX_SLOTS);
     if (tw != TW || th != TH) sized = false;
     TW = tw;
     TH = th;
@@ -704,9 +711,9 @@ int TiledPlan::set_tile(uint32_t tw, uint32_t th)
 
 size_t TiledPlan::raster_lds() const
 {
+    // keys (8 B) and staged cell + depth (8 B) per slot, hash table, triangle rows
     size_t S = (size_t)(TW / 2) * TH;
-    size_t fast = (size_t)HASH_CAP * 8 + (size_t)CACHE_CAP * CACHE_WORDS * 4;
-    return S * 8 + std::max(fast, S * 8);
+    return S * 16 + (size_t)HASH_CAP * 8 + (size_t)CHUNK * CACHE_WORDS * 4;
 }
 
 int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
@@ -726,7 +733,7 @@ int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
     if ((rc = d_off.reserve(sizeof(uint64_t) * (n_tiles + 1)))) return rc;
     if ((rc = d_nl.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_rows)))) return rc;
     if ((rc = d_overflow.reserve(sizeof(uint32_t) * std::max<size_t>(1, frames.size())))) return rc;
-    if ((rc = d_careful_cnt.reserve(sizeof(uint32_t)))) return rc;
+    if ((rc = d_careful_cnt.reserve(sizeof(uint32_t) * (2 + MAX_GROUPS)))) return rc;
     return SLOTH_OK;
 }
 
@@ -771,7 +778,8 @@ int TiledPlan::launch_bin(const std::vector<Frame>& frames, const std::vector<Dr
     return SLOTH_OK;
 }
 
-int TiledPlan::launch_careful(const Frame* d_frames, const Draw* d_draws, bool count_only, hipStream_t s)
+int TiledPlan::launch_careful(const Frame* d_frames, const Draw* d_draws, bool count_only, uint32_t group,
+                              hipStream_t s)
 {
     if (careful_cap == 0) return SLOTH_OK;
     const int tws = log2u(TW), ths = log2u(TH);
@@ -780,12 +788,12 @@ int TiledPlan::launch_careful(const Frame* d_frames, const Draw* d_draws, bool c
     if (count_only)
         hipLaunchKernelGGL(k_careful<true>, grid, dim3(256), 0, s, d_draws, d_frames, tws, ths,
                            d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
-                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), d_careful_in.as<uint2>(),
+                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), group, d_careful_in.as<uint2>(),
                            d_careful_box.as<uint4>(), careful_cap);
     else
         hipLaunchKernelGGL(k_careful<false>, grid, dim3(256), 0, s, d_draws, d_frames, tws, ths,
                            d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
-                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), d_careful_in.as<uint2>(),
+                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), group, d_careful_in.as<uint2>(),
                            d_careful_box.as<uint4>(), careful_cap);
     HIP_TRY(hipGetLastError());
     return SLOTH_OK;
@@ -812,7 +820,7 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
         if ((rc = launch_bin(frames, draws, d_frames, d_draws, 0, F, true, s))) return rc;
     }
     // 3. the careful triangles' tile entries
-    if ((rc = launch_careful(d_frames, d_draws, true, s))) return rc;
+    if ((rc = launch_careful(d_frames, d_draws, true, 0, s))) return rc;
     std::vector<uint32_t> counts(n_tiles);
     if (n_tiles)
         HIP_TRY(hipMemcpyAsync(counts.data(), d_cursor.p, sizeof(uint32_t) * n_tiles, hipMemcpyDeviceToHost, s));
@@ -831,31 +839,68 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     return SLOTH_OK;
 }
 
+int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_frames, const Draw* d_draws,
+                             const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells, float* z, int32_t* win,
+                             bool win_raw, uint32_t n_ramps, hipStream_t s)
+{
+    // grid (tile x, tile y, frame) over the largest tile grid of the range,
+    // at most 65535 frames per launch
+    for (uint32_t f0 = f_lo; f0 < f_hi; f0 += 65535u) {
+        const uint32_t f1 = std::min(f_hi, f0 + 65535u);
+        uint32_t mx = 0, my = 0;
+        for (uint32_t f = f0; f < f1; ++f) {
+            mx = std::max(mx, frames[f].tiles_x);
+            my = std::max(my, frames[f].tiles_y);
+        }
+        if (mx == 0 || my == 0) continue;
+        hipLaunchKernelGGL(k_tile_raster, dim3(mx, my, f1 - f0), dim3(RASTER_BLOCK), raster_lds(), s, d_draws, d_ramps,
+                           d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(), d_off.as<uint64_t>(),
+                           d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_careful_box.as<uint4>(), cells, z, win,
+                           win_raw ? 1u : 0u, n_ramps);
+        HIP_TRY(hipGetLastError());
+    }
+    return SLOTH_OK;
+}
+
 int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>& draws, const Frame* d_frames,
                       const Draw* d_draws, const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells,
                       float* z, int32_t* win, bool win_raw, uint32_t n_ramps, hipStream_t s,
-                      std::vector<hipEvent_t>* ev)
+                      std::vector<hipEvent_t>* ev, hipStream_t s2, const std::vector<hipEvent_t>* join)
 {
     if (!sized) return fail(SLOTH_ESTATE, "tile lists not sized");
     int rc;
-    if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
-    if ((rc = launch_clear(frames, f_lo, f_hi, s))) return rc;
-    if (ev) HIP_TRY(hipEventRecord((*ev)[1], s));
-    if ((rc = launch_bin(frames, draws, d_frames, d_draws, f_lo, f_hi, false, s))) return rc;
-    if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
-    if ((rc = launch_careful(d_frames, d_draws, false, s))) return rc;
-    if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
-    const Frame& a = frames[f_lo];
-    const Frame& b = frames[f_hi - 1];
-    uint64_t t_lo = a.tile_begin, t_hi = b.tile_begin + (uint64_t)b.tiles_x * b.tiles_y;
-    if (t_hi > t_lo) {
-        hipLaunchKernelGGL(k_tile_raster, dim3((unsigned)(t_hi - t_lo)), dim3(RASTER_BLOCK), raster_lds(), s, d_draws,
-                           d_ramps, d_frames, f_lo, f_hi, t_lo, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
-                           d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_careful_box.as<uint4>(),
-                           cells, z, win, win_raw ? 1u : 0u, n_ramps);
-        HIP_TRY(hipGetLastError());
+    const uint32_t nf = f_hi - f_lo;
+    uint32_t groups = 1;
+    if (s2 && join && !ev) groups = std::min<uint32_t>({MAX_GROUPS, nf / PIPE_MIN_FRAMES, (uint32_t)join->size() - 1});
+    if (groups < 2) {
+        if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
+        if ((rc = launch_clear(frames, f_lo, f_hi, s))) return rc;
+        if (ev) HIP_TRY(hipEventRecord((*ev)[1], s));
+        if ((rc = launch_bin(frames, draws, d_frames, d_draws, f_lo, f_hi, false, s))) return rc;
+        if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
+        if ((rc = launch_careful(d_frames, d_draws, false, 0, s))) return rc;
+        if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
+        if ((rc = launch_raster(frames, d_frames, d_draws, d_ramps, f_lo, f_hi, cells, z, win, win_raw, n_ramps, s)))
+            return rc;
+        if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));
+        return SLOTH_OK;
     }
-    if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));
+    // Pipeline over frame groups: s bins group g + 1 while s2 rasters group
+    // g.  Group g's raster waits for its bins and careful boxes (event g); s
+    // joins s2 at the end.
+    if ((rc = launch_clear(frames, f_lo, f_hi, s))) return rc;
+    for (uint32_t g = 0; g < groups; ++g) {
+        const uint32_t a = f_lo + (uint32_t)((uint64_t)nf * g / groups);
+        const uint32_t b = f_lo + (uint32_t)((uint64_t)nf * (g + 1) / groups);
+        if ((rc = launch_bin(frames, draws, d_frames, d_draws, a, b, false, s))) return rc;
+        if ((rc = launch_careful(d_frames, d_draws, false, g, s))) return rc;
+        HIP_TRY(hipEventRecord((*join)[g], s));
+        HIP_TRY(hipStreamWaitEvent(s2, (*join)[g], 0));
+        if ((rc = launch_raster(frames, d_frames, d_draws, d_ramps, a, b, cells, z, win, win_raw, n_ramps, s2)))
+            return rc;
+    }
+    HIP_TRY(hipEventRecord((*join)[groups], s2));
+    HIP_TRY(hipStreamWaitEvent(s, (*join)[groups], 0));
     return SLOTH_OK;
 }
 

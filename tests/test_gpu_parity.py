@@ -76,8 +76,12 @@ def test_custom_shaders_per_draw(sloth):
     U = meshgen.screen_transform(W, H, [np.array([1.0, 1.0, 1.0], np.float32)])
     sh1 = Shader((0.5, 0.9), "ab", "z")
     sh2 = Shader((-0.1, 0.25, 0.75, 1.0, 2.0), "01234", "#")
+    # not non-decreasing, and with a repeated threshold: the first-match scan
+    sh3 = Shader((0.9, 0.3, 0.6, 0.3, 0.95), "vwxyz", "q")
+    # non-decreasing with repeats and an infinite step
+    sh4 = Shader((0.25, 0.25, 0.5, float("inf")), "abcd", "e")
     draws = []
-    for sh in (sh1, sh2, None):
+    for sh in (sh1, sh2, sh3, sh4, None):
         xyz, rgb = random_soup(rng, 300, W, H, U, size=0.1)
         draws.append((xyz, rgb, meshgen.rotation_euler(0.2, 1.1, 0.3), sh))
     _check(sloth, "per-draw shaders", W, H, U, draws)
@@ -219,8 +223,58 @@ def test_batch_matches_oracle(sloth):
         for f, fs in enumerate(frames):
             fd = [(meshes[mi].xyz, meshes[mi].rgb, R, None) for (ff, mi, R) in plan if ff == f]
             ref = oracle_render(fs.width, fs.height, fs.utransform, fd, image=fs.image)
+            # cells and depths as the launch left them (asking for winners re-renders the frame)
+            cells, z, _ = b.read_frame(f)
+            assert_frame_equal(cells, z, None, ref, fs.width)
             cells, z, win = b.read_frame(f, want_winners=True)
             assert_frame_equal(cells, z, win, ref, fs.width)
+
+
+def _slivers(rng, n, spread=1.0):
+    """Nearly collinear triangles: the box bound cannot place them, so their
+    candidate boxes come from exact per-row spans."""
+    a = rng.uniform(-spread, spread, size=(n, 3))
+    d = rng.normal(0.0, 0.3, size=(n, 3))
+    perp = rng.normal(0.0, 1.0, size=(n, 3)) * rng.choice([1e-3, 1e-4, 1e-5, 0.0], size=(n, 1))
+    xyz = np.stack([a, a + d, a + 0.5 * d + perp], 1).astype(np.float32)
+    return xyz, rng.integers(0, 256, size=(n, 3), dtype=np.uint8)
+
+
+@pytest.mark.parametrize("n_frames,pipeline", [(40, True), (136, True), (136, False)])
+def test_batch_pipelined_groups(sloth, n_frames, pipeline, monkeypatch):
+    """With SLOTH_PIPELINE=1, graph launches of 32+ frames bin one frame group
+    while rastering the previous one; each group's careful triangles (slivers)
+    come from its own range of the shared careful list.  Every frame against
+    the oracle, on two replays of the graph."""
+    from sloth_amd import meshgen
+
+    monkeypatch.setenv("SLOTH_PIPELINE", "1" if pipeline else "0")
+    rng = np.random.default_rng(100 + n_frames)
+    W, H = 96, 40
+    U = meshgen.screen_transform(W, H, [np.array([1.0, 1.0, 1.0], np.float32)])
+    soup = sloth.SimpleMesh(*random_soup(rng, 300, W, H, U, size=0.06))
+    sliv = sloth.SimpleMesh(*_slivers(rng, 40))
+    frames = [sloth.FrameSpec(W, H, U, image=(f % 3 != 1)) for f in range(n_frames)]
+    draws, plan = [], []
+    for f in range(n_frames):
+        if f % 11 == 5:
+            continue   # a frame with no draw
+        for mesh in ((soup, sliv) if f % 2 == 0 else (sliv, soup, sliv)):
+            R = meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3))
+            draws.append(sloth.DrawSpec(f, mesh, R))
+            plan.append((f, mesh, R))
+    b = sloth.Batch(frames, draws)
+    refs = []
+    for f, fs in enumerate(frames):
+        fd = [(m.positions, m.colors, R, None) for (ff, m, R) in plan if ff == f]
+        refs.append(oracle_render(W, H, U, fd, image=fs.image))
+    for rep in range(2):
+        say(f"case pipelined batch {n_frames} frames, replay {rep}")
+        b.run()
+        b.sync()
+        for f in range(n_frames):
+            cells, z, _ = b.read_frame(f)
+            assert_frame_equal(cells, z, None, refs[f], W)
 
 
 def test_context_redraw_after_read(sloth):
