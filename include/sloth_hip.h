@@ -91,6 +91,30 @@ int sloth_mesh_bbox(const sloth_mesh *mesh, float *min3, float *max3);
 uint32_t sloth_mesh_triangles(const sloth_mesh *mesh);
 int sloth_mesh_destroy(sloth_mesh *mesh);
 
+/* ---- model files ----------------------------------------------------------
+ * The mesh queue of match_meshes (inputs.rs:93-129) for one file, parsed on
+ * the host: ".obj" (any case) as tobj 3.2.2 loads it with GPU_LOAD_OPTIONS
+ * (inputs.rs:108) and to_simple_mesh_with_materials converts it
+ * (geometry.rs:83-138): one mesh per model, faces fan-triangulated, colour =
+ * material Kd * 255 as u8, or the first corner's vertex colour, or (1,1,1)
+ * without materials; bounding box from the origin.  ".stl" as stl_io 0.4.2's
+ * read_stl (inputs.rs:117) and to_simple_mesh (geometry.rs:146-181): ASCII or
+ * binary, colour (255,255,0), bounding box from (f32::MAX, f32::MIN).  Files
+ * on which the reference panics (a named material library that cannot be
+ * read, a model without a material while materials exist, bad indices)
+ * return SLOTH_EINVAL. */
+typedef struct sloth_model sloth_model;
+int sloth_model_load(const char *path, sloth_model **out);
+int sloth_model_mesh_count(const sloth_model *model, uint32_t *n_meshes);
+/* Mesh i's soup (positions [n][3][3], colours [n][3]; valid until the model
+ * is destroyed), triangle count and bounding box; any output may be NULL. */
+int sloth_model_mesh(const sloth_model *model, uint32_t i, const float **positions, const uint8_t **colors,
+                     uint32_t *n_tris, float *bbox_min3, float *bbox_max3);
+/* Mesh i uploaded as a device mesh; sloth_mesh_bbox then reports the
+ * loader's bounding box. */
+int sloth_model_upload(const sloth_model *model, uint32_t i, sloth_mesh **out);
+int sloth_model_destroy(sloth_model *model);
+
 /* ---- Context (context.rs:12-142) -------------------------------------- */
 
 /* Context::blank (context.rs:22-34) followed by a fixed size, as image mode's
@@ -138,13 +162,18 @@ int sloth_context_device_buffers(sloth_context *ctx, uint32_t **d_cells, float *
 /* Context::flush's byte streams (context.rs:50-92), encoded on the device:
  *   SLOTH_ENCODE_TEXT  the no-colour flush (context.rs:58-61): every cell's
  *                      char, then println!'s '\n'
+ *   SLOTH_ENCODE_ANSI  the colour terminal flush (context.rs:62-78): per
+ *                      cell crossterm 0.18.0's PrintStyledContent of the char
+ *                      with its RGB on RGB (25,25,25): ESC "[48;2;25;25;25m",
+ *                      ESC "[38;2;R;G;Bm", the char, ESC "[0m"
  *   SLOTH_ENCODE_HTML  the -j export (context.rs:79-87): per cell
  *                      `<span style="color:rgb(R,G,B)">` then the char
- * Chars are written as UTF-8 (a glyph byte is a code point below 256).  The
- * ANSI truecolour flush (context.rs:62-77) is not provided: its bytes are
- * crossterm's.  With out = NULL only *len is set; otherwise out must hold
- * cap >= *len bytes.  Renders first if needed. */
+ * A terminal-mode frame (image = 0) starts with crossterm's MoveTo(0, 0),
+ * ESC "[1;1H" (context.rs:53-55).  Chars are written as UTF-8 (a glyph byte is
+ * a code point below 256).  With out = NULL only *len is set; otherwise out
+ * must hold cap >= *len bytes.  Renders first if needed. */
 #define SLOTH_ENCODE_TEXT 0
+#define SLOTH_ENCODE_ANSI 1
 #define SLOTH_ENCODE_HTML 2
 int sloth_context_encode(sloth_context *ctx, int mode, char *out, uint64_t cap, uint64_t *len);
 

@@ -68,7 +68,7 @@ def lib():
         L.oracle_render_frames.argtypes = [C.c_uint32, C.c_uint32, C.c_int, fp, fp, C.c_uint32, fp,
                                            u8p, C.c_uint32, u32p, fp, C.POINTER(Ramp), C.c_int]
         L.oracle_render_frames.restype = C.c_int
-        L.oracle_encode.argtypes = [u32p, C.c_uint64, C.c_int, C.c_char_p, C.c_size_t]
+        L.oracle_encode.argtypes = [u32p, C.c_uint64, C.c_int, C.c_int, C.c_char_p, C.c_size_t]
         L.oracle_encode.restype = C.c_size_t
         L.oracle_shader_exhaustive_mismatch.argtypes = [C.c_uint32, C.c_uint32]
         L.oracle_shader_exhaustive_mismatch.restype = C.c_uint32
@@ -252,11 +252,13 @@ def shade_f64(frame: Frame, M, xyz, faces, vcolor, vnormal, light, target=None, 
     return float(loss), out, grads
 
 
-def encode(cells, mode: int) -> bytes:
-    """Context::flush's bytes: mode 0 plain text (context.rs:58-61), mode 2
-    the -j HTML spans (context.rs:79-87)."""
+def encode(cells, mode: int, image: bool = True) -> bytes:
+    """Context::flush's bytes: mode 0 plain text (context.rs:58-61), mode 1
+    the colour terminal stream (context.rs:62-78), mode 2 the -j HTML spans
+    (context.rs:79-87); a terminal-mode frame (image False) starts with the
+    cursor move (context.rs:53-55)."""
     c = np.ascontiguousarray(cells, np.uint32)
-    n = lib().oracle_encode(_p(c, C.c_uint32), c.size, mode, None, 0)
+    n = lib().oracle_encode(_p(c, C.c_uint32), c.size, mode, int(bool(image)), None, 0)
     buf = C.create_string_buffer(max(1, n))
-    lib().oracle_encode(_p(c, C.c_uint32), c.size, mode, buf, n)
+    lib().oracle_encode(_p(c, C.c_uint32), c.size, mode, int(bool(image)), buf, n)
     return buf.raw[:n]

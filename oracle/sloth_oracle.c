@@ -547,27 +547,46 @@ static size_t put_char_utf8(char *p, uint32_t g)
     return 2;
 }
 
-/* mode 0: `let frame: String = frame_buffer.iter().map(|p| p.0).collect();
- * println!("{}", frame);` (context.rs:58-61).  mode 2: for every cell
+/* Context::flush (context.rs:50-92).  A terminal-mode context (image = 0)
+ * first queues crossterm's MoveTo(0, 0) = CSI "1;1H" (context.rs:53-55).
+ * mode 0: `let frame: String = frame_buffer.iter().map(|p| p.0).collect();
+ * println!("{}", frame);` (context.rs:58-61).  mode 1: per cell crossterm
+ * 0.18.0's PrintStyledContent of style(char).with(Rgb fg).on(Rgb 25,25,25)
+ * (context.rs:62-78): background CSI "48;2;25;25;25m", foreground CSI
+ * "38;2;R;G;Bm", the char, then ResetColor CSI "0m" (crossterm's
+ * StyledContent Display; the crate is not vendored, no fixture pins these
+ * bytes).  mode 2: for every cell
  * print!("<span style=\"color:rgb({},{},{})\">{}", r, g, b, char)
  * (context.rs:79-87).  Writes into out (if non-NULL, cap bytes) and returns
  * the full length. */
-size_t oracle_encode(const uint32_t *cells, uint64_t n, int mode, char *out, size_t cap)
+size_t oracle_encode(const uint32_t *cells, uint64_t n, int mode, int image, char *out, size_t cap)
 {
     size_t len = 0;
-    char tmp[64];
+    char tmp[96];
+    if (!image) {
+        const char *mv = "\x1b[1;1H";
+        if (out && len + 6 <= cap) memcpy(out + len, mv, 6);
+        len += 6;
+    }
     for (uint64_t i = 0; i < n; ++i) {
         uint32_t c = cells[i];
-        if (mode == 2) {
-            int k = snprintf(tmp, sizeof tmp, "<span style=\"color:rgb(%u,%u,%u)\">", (c >> 8) & 0xFFu,
-                             (c >> 16) & 0xFFu, c >> 24);
-            if (out && len + (size_t)k <= cap) memcpy(out + len, tmp, (size_t)k);
-            len += (size_t)k;
-        }
+        int k = 0;
+        if (mode == 1)
+            k = snprintf(tmp, sizeof tmp, "\x1b[48;2;25;25;25m\x1b[38;2;%u;%u;%um", (c >> 8) & 0xFFu,
+                         (c >> 16) & 0xFFu, c >> 24);
+        else if (mode == 2)
+            k = snprintf(tmp, sizeof tmp, "<span style=\"color:rgb(%u,%u,%u)\">", (c >> 8) & 0xFFu,
+                         (c >> 16) & 0xFFu, c >> 24);
+        if (out && len + (size_t)k <= cap) memcpy(out + len, tmp, (size_t)k);
+        len += (size_t)k;
         char u[2];
-        size_t k = put_char_utf8(u, c & 0xFFu);
-        if (out && len + k <= cap) memcpy(out + len, u, k);
-        len += k;
+        size_t ku = put_char_utf8(u, c & 0xFFu);
+        if (out && len + ku <= cap) memcpy(out + len, u, ku);
+        len += ku;
+        if (mode == 1) {
+            if (out && len + 4 <= cap) memcpy(out + len, "\x1b[0m", 4);
+            len += 4;
+        }
     }
     if (mode == 0) {
         if (out && len < cap) out[len] = '\n';

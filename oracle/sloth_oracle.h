@@ -42,7 +42,7 @@ int oracle_render_frames(uint32_t W, uint32_t H, int image, const float *U, cons
                          uint32_t *cells_out, float *z_out, const oracle_ramp *ramp, int nthreads);
 uint32_t oracle_shader_exhaustive_mismatch(uint32_t lo_hi16, uint32_t hi_hi16);
 #include <stddef.h>
-size_t oracle_encode(const uint32_t *cells, uint64_t n, int mode, char *out, size_t cap);
+size_t oracle_encode(const uint32_t *cells, uint64_t n, int mode, int image, char *out, size_t cap);
 
 #ifdef __cplusplus
 }

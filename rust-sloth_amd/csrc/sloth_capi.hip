@@ -19,6 +19,7 @@
 #include "sloth_host.h"
 #include "sloth_encode.h"
 #include "sloth_kernels.h"
+#include "sloth_models.h"
 #include "sloth_shade.h"
 #include "sloth_tiled.h"
 
@@ -413,10 +414,15 @@ static hipStream_t make_stream()
 struct Encoder {
     DevBuf d_off, d_block, d_total, d_out;
 
-    int run(const uint32_t* d_cells, uint64_t n, int mode, char* out, uint64_t cap, uint64_t* len, hipStream_t s)
+    // n cells of one frame; a terminal-mode frame (image false) starts with
+    // crossterm's MoveTo(0, 0), CSI "1;1H" (context.rs:53-55)
+    int run(const uint32_t* d_cells, uint64_t n, bool image, int mode, char* out, uint64_t cap, uint64_t* len,
+            hipStream_t s)
     {
-        if (mode != SLOTH_ENCODE_TEXT && mode != SLOTH_ENCODE_HTML)
-            return fail(SLOTH_EINVAL, "encode mode must be SLOTH_ENCODE_TEXT or SLOTH_ENCODE_HTML");
+        if (mode != SLOTH_ENCODE_TEXT && mode != SLOTH_ENCODE_ANSI && mode != SLOTH_ENCODE_HTML)
+            return fail(SLOTH_EINVAL, "encode mode must be SLOTH_ENCODE_TEXT, SLOTH_ENCODE_ANSI or SLOTH_ENCODE_HTML");
+        static const char MOVE_TO_ORIGIN[] = "\x1b[1;1H";
+        const uint64_t pre = image ? 0 : sizeof MOVE_TO_ORIGIN - 1;
         int rc;
         const uint64_t nb = encode_blocks(n);
         if ((rc = d_off.reserve(std::max<uint64_t>(8, n * 8))) || (rc = d_block.reserve(std::max<uint64_t>(8, nb * 8))) ||
@@ -430,20 +436,21 @@ struct Encoder {
             HIP_TRY(hipMemcpyAsync(&total, d_total.p, 8, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
-        const uint64_t full = total + (mode == SLOTH_ENCODE_TEXT ? 1 : 0);   // println!'s newline
+        const uint64_t full = pre + total + (mode == SLOTH_ENCODE_TEXT ? 1 : 0);   // println!'s newline
         if (len) *len = full;
         if (!out) return SLOTH_OK;
         if (cap < full) return fail(SLOTH_EINVAL, "encode buffer holds %llu bytes, %llu needed",
                                     (unsigned long long)cap, (unsigned long long)full);
+        memcpy(out, MOVE_TO_ORIGIN, pre);
         if (total) {
             if ((rc = d_out.reserve(total))) return rc;
             if ((rc = launch_encode_write(d_cells, n, mode, d_off.as<uint64_t>(), d_block.as<uint64_t>(),
                                           d_out.as<uint8_t>(), s)))
                 return rc;
-            HIP_TRY(hipMemcpyAsync(out, d_out.p, total, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(out + pre, d_out.p, total, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
-        if (mode == SLOTH_ENCODE_TEXT) out[total] = '\n';
+        if (mode == SLOTH_ENCODE_TEXT) out[pre + total] = '\n';
         return SLOTH_OK;
     }
 };
@@ -632,6 +639,77 @@ int sloth_mesh_bbox(const sloth_mesh* mesh, float* min3, float* max3)
 
 uint32_t sloth_mesh_triangles(const sloth_mesh* mesh) { return mesh ? mesh->n : 0; }
 
+// ---- model files (inputs.rs:93-129, geometry.rs:83-181) ----------------------
+
+}  // extern "C"
+
+struct sloth_model {
+    std::vector<SoupMesh> meshes;
+};
+
+extern "C" {
+
+int sloth_model_load(const char* path, sloth_model** out)
+{
+    ARG_CHECK(path && out, "null argument");
+    *out = nullptr;
+    sloth_model* m = new (std::nothrow) sloth_model();
+    if (!m) return fail(SLOTH_ENOMEM, "host allocation failed");
+    std::string err;
+    try {
+        err = load_model_file(path, m->meshes);
+    } catch (const std::bad_alloc&) {
+        delete m;
+        return fail(SLOTH_ENOMEM, "host allocation failed while loading %s", path);
+    }
+    if (!err.empty()) {
+        delete m;
+        return fail(SLOTH_EINVAL, "%s: %s", path, err.c_str());
+    }
+    *out = m;
+    return SLOTH_OK;
+}
+
+int sloth_model_mesh_count(const sloth_model* model, uint32_t* n_meshes)
+{
+    ARG_CHECK(model && n_meshes, "null argument");
+    *n_meshes = (uint32_t)model->meshes.size();
+    return SLOTH_OK;
+}
+
+int sloth_model_mesh(const sloth_model* model, uint32_t i, const float** positions, const uint8_t** colors,
+                     uint32_t* n_tris, float* bbox_min3, float* bbox_max3)
+{
+    ARG_CHECK(model, "null model");
+    ARG_CHECK(i < model->meshes.size(), "mesh index out of range");
+    const SoupMesh& m = model->meshes[i];
+    if (positions) *positions = m.xyz.data();
+    if (colors) *colors = m.rgb.data();
+    if (n_tris) *n_tris = m.n();
+    if (bbox_min3) memcpy(bbox_min3, m.bbmin, 12);
+    if (bbox_max3) memcpy(bbox_max3, m.bbmax, 12);
+    return SLOTH_OK;
+}
+
+int sloth_model_upload(const sloth_model* model, uint32_t i, sloth_mesh** out)
+{
+    ARG_CHECK(model && out, "null argument");
+    ARG_CHECK(i < model->meshes.size(), "mesh index out of range");
+    const SoupMesh& m = model->meshes[i];
+    int rc = sloth_mesh_create(m.xyz.data(), m.rgb.data(), m.n(), out);
+    if (rc) return rc;
+    // the loader's bounding-box rule (an STL box does not start at the origin)
+    memcpy((*out)->bbmin, m.bbmin, 12);
+    memcpy((*out)->bbmax, m.bbmax, 12);
+    return SLOTH_OK;
+}
+
+int sloth_model_destroy(sloth_model* model)
+{
+    delete model;
+    return SLOTH_OK;
+}
+
 int sloth_mesh_destroy(sloth_mesh* mesh)
 {
     delete mesh;
@@ -772,7 +850,10 @@ int sloth_context_encode(sloth_context* ctx, int mode, char* out, uint64_t cap, 
     ARG_CHECK(ctx, "null context");
     int rc = sloth_context_read(ctx, nullptr, nullptr, nullptr);   // render + overflow check
     if (rc) return rc;
-    return ctx->enc.run(ctx->job.d_cells.as<uint32_t>(), ctx->job.n_cells, mode, out, cap, len, ctx->stream);
+    // the frame's own cells (the layout pads the buffer to an even count)
+    const Frame& fr = ctx->job.frames[0];
+    const uint64_t nc = (uint64_t)fr.W * fr.H + (fr.image ? fr.H : 0);
+    return ctx->enc.run(ctx->job.d_cells.as<uint32_t>(), nc, fr.image != 0, mode, out, cap, len, ctx->stream);
 }
 
 int sloth_context_device_buffers(sloth_context* ctx, uint32_t** d_cells, float** d_z, uint64_t* n_cells,
@@ -976,7 +1057,7 @@ int sloth_batch_encode_frame(sloth_batch* b, uint32_t frame, int mode, char* out
     if (rc) return rc;
     const Frame& fr = b->job.frames[frame];
     const uint64_t nc = (uint64_t)fr.W * fr.H + (fr.image ? fr.H : 0);
-    return b->enc.run(b->job.d_cells.as<uint32_t>() + fr.cell_off, nc, mode, out, cap, len, b->stream);
+    return b->enc.run(b->job.d_cells.as<uint32_t>() + fr.cell_off, nc, fr.image != 0, mode, out, cap, len, b->stream);
 }
 
 int sloth_batch_device_frame(sloth_batch* b, uint32_t frame, uint32_t** d_cells, float** d_z)

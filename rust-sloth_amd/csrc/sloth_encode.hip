@@ -1,6 +1,9 @@
 // Output encoders of Context::flush (context.rs:50-92) on the device.
 //
 //   plain  (context.rs:58-61): every cell's char, then println!'s '\n'
+//   ansi   (context.rs:62-78): per cell crossterm 0.18.0's PrintStyledContent
+//          of the char with RGB foreground on RGB (25,25,25): CSI
+//          "48;2;25;25;25m", CSI "38;2;R;G;Bm", the char, CSI "0m"
 //   html   (context.rs:79-87): per cell `<span style="color:rgb(R,G,B)">` and
 //          the char (the `-j` export format)
 //
@@ -22,12 +25,16 @@ __device__ __forceinline__ uint32_t dec_len(uint32_t v) { return v >= 100u ? 3u 
 
 // `<span style="color:rgb(` = 23 bytes, `)">` = 3 bytes, plus 2 commas
 constexpr uint32_t HTML_FIXED = 23 + 3 + 2;
+// ESC "[48;2;25;25;25m" = 16 bytes, ESC "[38;2;" = 7, two ';' and 'm' = 3,
+// ESC "[0m" = 4
+constexpr uint32_t ANSI_FIXED = 16 + 7 + 3 + 4;
 
 __device__ __forceinline__ uint32_t cell_len(uint32_t c, int mode)
 {
     const uint32_t g = c & 0xFFu;
     if (mode == 0) return utf8_len(g);
-    return HTML_FIXED + dec_len((c >> 8) & 0xFFu) + dec_len((c >> 16) & 0xFFu) + dec_len(c >> 24) + utf8_len(g);
+    const uint32_t digits = dec_len((c >> 8) & 0xFFu) + dec_len((c >> 16) & 0xFFu) + dec_len(c >> 24);
+    return (mode == 1 ? ANSI_FIXED : HTML_FIXED) + digits + utf8_len(g);
 }
 
 // Per block: the exclusive scan of its cells' lengths (written to len) and
@@ -128,6 +135,23 @@ __global__ void __launch_bounds__(ENC_BLOCK) k_enc_write(const uint32_t* __restr
     uint8_t* p = out + block_off[i / ((uint64_t)ENC_BLOCK * SCAN_ITEMS)] + off[i];
     if (mode == 0) {
         put_utf8(p, c & 0xFFu);
+        return;
+    }
+    if (mode == 1) {
+        const char* bg = "\x1b[48;2;25;25;25m\x1b[38;2;";
+        uint32_t k = 0;
+        for (; k < 23; ++k) p[k] = (uint8_t)bg[k];
+        k += put_dec(p + k, (c >> 8) & 0xFFu);
+        p[k++] = ';';
+        k += put_dec(p + k, (c >> 16) & 0xFFu);
+        p[k++] = ';';
+        k += put_dec(p + k, c >> 24);
+        p[k++] = 'm';
+        k += put_utf8(p + k, c & 0xFFu);
+        p[k++] = 0x1b;
+        p[k++] = '[';
+        p[k++] = '0';
+        p[k++] = 'm';
         return;
     }
     const char* head = "<span style=\"color:rgb(";

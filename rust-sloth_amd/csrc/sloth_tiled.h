@@ -18,7 +18,7 @@
 //                  64-bit LDS atomicMin of the packed keys, then every slot
 //                  re-evaluates its winner from the LDS row and stores its two
 //                  cells and depths (coalesced).  Lists longer than the 256
-//                  LDS rows run in chunks, staging winners after each chunk.
+//                  LDS rows run in chunks, storing winners after each chunk.
 //
 // Tile lists live in one pool at fixed per-tile offsets, sized from an exact
 // counting run of k_bin plus headroom.  If a later run would exceed a tile's
@@ -38,7 +38,7 @@ constexpr uint32_t MAX_GROUPS = 8;        // frame groups of a pipelined launch
 constexpr uint32_t PIPE_MIN_FRAMES = 16;  // frames per group at least
 
 struct TiledPlan {
-    uint32_t TW = 64, TH = 16;       // tile size in cells (TW even)
+    uint32_t TW = 64, TH = 32;       // tile size in cells (TW even)
     uint64_t n_tiles = 0;
     uint64_t n_rows = 0;
     bool sized = false;              // pool offsets match the current layout
@@ -54,7 +54,7 @@ struct TiledPlan {
     // state.  Pool offsets become invalid until size() runs.
     int plan(std::vector<Frame>& frames, uint64_t rows);
 
-    // Tile size in cells: TW a power of two in 8..128, TH in 4..64, TW * TH <= 4096.
+    // Tile size in cells: TW a power of two in 8..128, TH in 4..64.
     int set_tile(uint32_t tw, uint32_t th);
     size_t raster_lds() const;
 

@@ -416,7 +416,7 @@ constexpr uint32_t CHUNK = RASTER_BLOCK;   // list entries whose triangles sit i
 constexpr uint32_t HASH_CAP = 512;         // open-addressing table: triangle -> cache row
 constexpr uint32_t CACHE_WORDS = 12;
 constexpr uint32_t LDS_RAMPS = 4;          // shaders staged in LDS per tile
-constexpr uint32_t MAX_SLOTS = 2048;       // (TW / 2) * TH: bounds the workgroup's LDS
+static_assert(CHUNK <= 256, "hash rows hold a cache row index in one byte");
 
 __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws, const Frame& fr, uint32_t tri)
 {
@@ -478,9 +478,10 @@ __device__ __forceinline__ uint32_t entry_tri(uint32_t e, const uint4* __restric
 // triangle goes to an LDS row found through a hash of its triangle index, and
 // its fragments take a 64-bit LDS atomicMin of the packed (depth, triangle,
 // sub) keys.  With one chunk, every slot then re-evaluates its winner from
-// the LDS row.  With several, after each chunk the slots whose key belongs to
-// one of the chunk's triangles stage their cell and depth (the last staging of
-// a slot is its final winner's, since a later winner comes from a later chunk).
+// the LDS row and stores it.  With several, after each chunk the slots whose
+// key belongs to one of the chunk's triangles store that winner (the last
+// store of a slot is its final winner's, since a later winner comes from a
+// later chunk), and the empty slots are stored at the end.
 // Tile (tx, ty) of frame fr, whose list is pool[o, o + n).
 __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, const Ramp* __restrict__ ramps,
                                             const Ramp* s_ramp, const Frame& fr, uint32_t tx, uint32_t ty, uint64_t o,
@@ -545,11 +546,9 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
     } else {
         auto ramp_of = [&](uint32_t r) -> const Ramp& { return r < LDS_RAMPS ? s_ramp[r] : ramps[r]; };
         uint64_t* keys = smem;
-        uint32_t* st_cell = (uint32_t*)(keys + S);
-        float* st_z = (float*)(st_cell + S);
-        uint32_t* htag = (uint32_t*)(st_z + S);
-        uint32_t* hrow = htag + HASH_CAP;
-        float* cache = (float*)(hrow + HASH_CAP);
+        uint32_t* htag = (uint32_t*)(keys + S);
+        float* cache = (float*)(htag + HASH_CAP);
+        uint8_t* hrow = (uint8_t*)(cache + CHUNK * CACHE_WORDS);   // cache row (< CHUNK) of a hash slot
         const bool one_draw = fr.draw_end - fr.draw_begin == 1;
         const bool multi = n > CHUNK;
 
@@ -605,7 +604,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 uint32_t h = hash_tri(T);
                 for (uint32_t probe = 0; probe < HASH_CAP; ++probe) {
                     if (atomicCAS(&htag[h], EMPTY32, T) == EMPTY32) {
-                        hrow[h] = p;
+                        hrow[h] = (uint8_t)p;
                         break;
                     }
                     h = (h + 1) & (HASH_CAP - 1);
@@ -626,30 +625,23 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             }
             __syncthreads();
             if (multi)
+                // slots whose key now belongs to this chunk store that winner
+                // (a later chunk that changes the key stores again; the same
+                // thread owns slot s throughout, so its last store lands last)
                 for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
                     const uint64_t k = keys[s];
                     uint32_t cell;
                     float z;
-                    if (k != KEY_EMPTY && winner(s, k, cell, z)) {
-                        st_cell[s] = cell;
-                        st_z[s] = z;
-                    }
+                    if (k != KEY_EMPTY && winner(s, k, cell, z)) store(s, k, cell, z);
                 }
         }
-        // the same thread staged slot s, so no barrier is needed before reading st_*
         for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
             if (!slot_valid(s)) continue;
             const uint64_t k = keys[s];
+            if (multi && k != KEY_EMPTY) continue;   // stored after its chunk
             uint32_t cell = CELL_BLANK;
             float z = F32_MAX;
-            if (k != KEY_EMPTY) {
-                if (multi) {
-                    cell = st_cell[s];
-                    z = st_z[s];
-                } else {
-                    winner(s, k, cell, z);
-                }
-            }
+            if (k != KEY_EMPTY) winner(s, k, cell, z);
             store(s, k, cell, z);
         }
     }
@@ -699,10 +691,8 @@ static int log2u(uint32_t v)
 
 int TiledPlan::set_tile(uint32_t tw, uint32_t th)
 {
-    if (tw < 8 || tw > 128 || th < 4 || th > 64 || (tw & (tw - 1)) || (th & (th - 1)) || (tw / 2) * th > MAX_SLOTS)
-        return fail(SLOTH_EINVAL,
-                    "tile size %ux%u: width must be a power of two in 8..128, height in 4..64, width * height <= %u",
-                    tw, th, 2 * MAX_SLOTS);
+    if (tw < 8 || tw > 128 || th < 4 || th > 64 || (tw & (tw - 1)) || (th & (th - 1)))
+        return fail(SLOTH_EINVAL, "tile size %ux%u: width must be a power of two in 8..128, height in 4..64", tw, th);
     if (tw != TW || th != TH) sized = false;
     TW = tw;
     TH = th;
@@ -711,9 +701,9 @@ int TiledPlan::set_tile(uint32_t tw, uint32_t th)
 
 size_t TiledPlan::raster_lds() const
 {
-    // keys (8 B) and staged cell + depth (8 B) per slot, hash table, triangle rows
+    // keys (8 B per slot), hash tags (4 B) and rows (1 B), triangle rows
     size_t S = (size_t)(TW / 2) * TH;
-    return S * 16 + (size_t)HASH_CAP * 8 + (size_t)CHUNK * CACHE_WORDS * 4;
+    return S * 8 + (size_t)HASH_CAP * 5 + (size_t)CHUNK * CACHE_WORDS * 4;
 }
 
 int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)

@@ -27,7 +27,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libsloth_hip.so")
 
 SLOTH_OK, SLOTH_EINVAL, SLOTH_ENOMEM, SLOTH_EHIP, SLOTH_ENODEV, SLOTH_ESTATE = range(6)
-ENCODE_TEXT, ENCODE_HTML = 0, 2
+ENCODE_TEXT, ENCODE_ANSI, ENCODE_HTML = 0, 1, 2
 
 
 def _encode(fn, *args) -> bytes:
@@ -76,6 +76,12 @@ SIGNATURES = {
     "sloth_mesh_bbox": (C.c_int, [_vp, _fp, _fp]),
     "sloth_mesh_triangles": (C.c_uint32, [_vp]),
     "sloth_mesh_destroy": (C.c_int, [_vp]),
+    "sloth_model_load": (C.c_int, [C.c_char_p, _vpp]),
+    "sloth_model_mesh_count": (C.c_int, [_vp, C.POINTER(C.c_uint32)]),
+    "sloth_model_mesh": (C.c_int, [_vp, C.c_uint32, C.POINTER(_fp), C.POINTER(_u8p), C.POINTER(C.c_uint32),
+                                   _fp, _fp]),
+    "sloth_model_upload": (C.c_int, [_vp, C.c_uint32, _vpp]),
+    "sloth_model_destroy": (C.c_int, [_vp]),
     "sloth_context_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, _vpp]),
     "sloth_context_destroy": (C.c_int, [_vp]),
     "sloth_context_update": (C.c_int, [_vp, _fp, C.c_uint32]),
@@ -235,6 +241,19 @@ class SimpleMesh:
     def from_mesh(cls, m) -> "SimpleMesh":
         return cls(m.xyz, m.rgb)
 
+    @classmethod
+    def _from_model(cls, model, i: int) -> "SimpleMesh":
+        L = load_library()
+        pos, rgb, bmin, bmax = model.mesh_arrays(i)
+        self = cls.__new__(cls)
+        _ensure_init()
+        self.positions, self.colors = pos, rgb
+        h = C.c_void_p()
+        _check(L.sloth_model_upload(model._h, i, C.byref(h)))
+        self._h = h
+        self.bounding_box = AABB(bmin, bmax)
+        return self
+
     @property
     def n(self) -> int:
         return int(self.positions.shape[0])
@@ -248,6 +267,54 @@ class SimpleMesh:
         if h is not None and _lib is not None:
             _lib.sloth_mesh_destroy(h)
             self._h = None
+
+
+class ModelFile:
+    """One model file parsed on the host (inputs.rs:93-129): `.obj` through
+    the tobj 3.2.2 rules, `.stl` through stl_io 0.4.2's; see
+    sloth_model_load.  No GPU is needed to parse."""
+
+    def __init__(self, path: str):
+        L = load_library()
+        h = C.c_void_p()
+        _check(L.sloth_model_load(os.fsencode(path), C.byref(h)))
+        self._h = h
+        n = C.c_uint32()
+        _check(L.sloth_model_mesh_count(h, C.byref(n)))
+        self.n_meshes = int(n.value)
+
+    def mesh_arrays(self, i: int):
+        """(positions [n,3,3] f32, colors [n,3] u8, bbox min, bbox max), copies."""
+        L = load_library()
+        p, c, n = _fp(), _u8p(), C.c_uint32()
+        mn = np.zeros(3, np.float32)
+        mx = np.zeros(3, np.float32)
+        _check(L.sloth_model_mesh(self._h, i, C.byref(p), C.byref(c), C.byref(n), _fptr(mn), _fptr(mx)))
+        k = int(n.value)
+        pos = np.ctypeslib.as_array(p, shape=(k * 9,)).reshape(k, 3, 3).copy() if k else np.zeros((0, 3, 3), np.float32)
+        rgb = np.ctypeslib.as_array(c, shape=(k * 3,)).reshape(k, 3).copy() if k else np.zeros((0, 3), np.uint8)
+        return pos, rgb, mn, mx
+
+    def meshes(self) -> list:
+        """The file's meshes as device SimpleMesh objects, in queue order."""
+        return [SimpleMesh._from_model(self, i) for i in range(self.n_meshes)]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.sloth_model_destroy(h)
+            self._h = None
+
+
+def load_meshes(paths) -> list:
+    """match_meshes (inputs.rs:93-129): the mesh queue of one or more files,
+    in order (a string splits on spaces, as the reference's argument does)."""
+    if isinstance(paths, str):
+        paths = paths.split(" ")
+    out = []
+    for p in paths:
+        out.extend(ModelFile(p).meshes())
+    return out
 
 
 # ---------------------------------------------------------------------------

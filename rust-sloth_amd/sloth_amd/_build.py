@@ -19,7 +19,8 @@ CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libsloth_hip.so")
 
-SOURCES = ["sloth_capi.hip", "sloth_global.hip", "sloth_tiled.hip", "sloth_shade.hip", "sloth_encode.hip"]
+SOURCES = ["sloth_capi.hip", "sloth_global.hip", "sloth_tiled.hip", "sloth_shade.hip", "sloth_encode.hip",
+           "sloth_models.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -22,6 +22,7 @@ def test_encode_sphere_frame(sloth):
     ref = oracle.render(W, H, U, m.xyz, m.rgb, R)
     say("case encode sphere 160x48")
     assert ctx.encode(sloth.ENCODE_TEXT) == oracle.encode(ref.cells, 0)
+    assert ctx.encode(sloth.ENCODE_ANSI) == oracle.encode(ref.cells, 1)
     assert ctx.encode(sloth.ENCODE_HTML) == oracle.encode(ref.cells, 2)
     text = ctx.encode(sloth.ENCODE_TEXT).decode()
     # newline cells (rows inside some triangle's domain) plus println!'s newline
@@ -43,7 +44,7 @@ def test_encode_non_ascii_glyphs(sloth):
     sloth.draw_mesh(ctx, sloth.SimpleMesh.from_mesh(m), R, sh)
     cells, _, _ = ctx.read()
     say("case encode non-ascii")
-    for mode in (sloth.ENCODE_TEXT, sloth.ENCODE_HTML):
+    for mode in (sloth.ENCODE_TEXT, sloth.ENCODE_ANSI, sloth.ENCODE_HTML):
         got = ctx.encode(mode)
         assert got == oracle.encode(cells, mode)
         got.decode("utf-8")
@@ -61,4 +62,28 @@ def test_encode_1080p_batch_frame(sloth):
     for f in (0, 3):
         cells, _, _ = b.read_frame(f)
         assert b.encode_frame(f, sloth.ENCODE_TEXT) == oracle.encode(cells, 0)
+        assert b.encode_frame(f, sloth.ENCODE_ANSI) == oracle.encode(cells, 1)
         assert b.encode_frame(f, sloth.ENCODE_HTML) == oracle.encode(cells, 2)
+
+
+def test_encode_terminal_mode_and_odd_cell_counts(sloth):
+    """A terminal-mode context starts with the cursor move; frames whose cell
+    count is odd (even W, odd H in image mode) encode exactly their own cells
+    (the device buffer is padded to an even count)."""
+    from sloth_amd import meshgen
+
+    m = meshgen.uv_sphere(16, 8, seed=4)
+    R = meshgen.rotation_euler(0.0, 0.7, 0.0)
+    for W, H, image in ((80, 25, True), (80, 25, False), (64, 21, True), (33, 17, False)):
+        U = meshgen.screen_transform(W, H, [m.bbox_max()])
+        ctx = sloth.Context(W, H, image=image)
+        ctx.utransform = U
+        ctx.clear()
+        sloth.draw_mesh(ctx, sloth.SimpleMesh.from_mesh(m), R)
+        cells, _, _ = ctx.read()
+        say(f"case encode {W}x{H} image={image}")
+        assert cells.size == W * H + (H if image else 0)
+        for mode in (sloth.ENCODE_TEXT, sloth.ENCODE_ANSI, sloth.ENCODE_HTML):
+            got = ctx.encode(mode)
+            assert got == oracle.encode(cells, mode, image=image)
+            assert got.startswith(b"\x1b[1;1H") == (not image)

@@ -360,3 +360,34 @@ def test_autotune_keeps_results(sloth, raster_path):
         ref = oracle_render(W, H, U, [(m.xyz, m.rgb, R, None)])
         cells, z, win = b.read_frame(k, want_winners=True)
         assert_frame_equal(cells, z, win, ref, W)
+
+
+def test_model_files_render(sloth):
+    """Meshes from the model loader (the reference's cube.obj and cube.stl)
+    uploaded, queued in file order and drawn after update, against the oracle
+    on the loader's own arrays; uploaded STL meshes keep the STL bounding box."""
+    import os
+    from sloth_amd import meshgen
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    models = os.path.join(here, "golden", "models")
+    queue = sloth.load_meshes([os.path.join(models, "cube.obj"), os.path.join(models, "cube.stl")])
+    assert len(queue) == 7
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    sloth.load_library().sloth_mesh_bbox(queue[-1].handle, lo.ctypes.data_as(sloth._fp), hi.ctypes.data_as(sloth._fp))
+    assert np.array_equal(hi, queue[-1].bounding_box.max) and np.array_equal(lo, queue[-1].bounding_box.min)
+    assert lo.min() < 0 and hi.max() > 30
+    W, H = 120, 40
+    U = meshgen.screen_transform(W, H, [m.bounding_box.max for m in queue])
+    R = meshgen.rotation_euler(0.3, 2.2, 0.1)
+    ctx = sloth.Context(W, H, image=True)
+    ctx.utransform = U
+    ctx.clear()
+    for m in queue:
+        sloth.draw_mesh(ctx, m, R)
+    cells, z, win = ctx.read(want_winners=True)
+    say("case model files cube.obj + cube.stl")
+    ref = oracle_render(W, H, U, [(m.positions, m.colors, R, None) for m in queue])
+    assert_frame_equal(cells, z, win, ref, W)
+    assert ref.stats[2] > 0

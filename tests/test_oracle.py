@@ -326,3 +326,25 @@ def test_shade_reduces_to_flat_shade():
         n_s = np.cross(t[1, :3] - t[0, :3], t[2, :3] - t[0, :3])
         flat = n_s[2] / np.linalg.norm(n_s) * sum(w) / o
         assert abs(out[idx, 0] - flat) < 1e-6
+
+
+def test_encode_known_answers():
+    """Context::flush bytes for three hand-made cells, written out from the
+    format strings of context.rs:58-61 (println! of the chars), :62-78
+    (crossterm 0.18.0 PrintStyledContent: background, foreground, char,
+    reset) and :79-87 (the -j spans), and the terminal-mode MoveTo(0, 0)
+    (:53-55).  Glyph bytes are chars below U+0100, so 0xE9 is 'é' (2 UTF-8
+    bytes)."""
+    cells = np.array([ord("@") | (255 << 8) | (0 << 16) | (7 << 24),
+                      0x0A,
+                      0xE9 | (1 << 8) | (22 << 16) | (133 << 24)], np.uint32)
+    assert oracle.encode(cells, 0) == b"@\n\xc3\xa9\n"
+    assert oracle.encode(cells, 0, image=False) == b"\x1b[1;1H@\n\xc3\xa9\n"
+    assert oracle.encode(cells, 2) == (b'<span style="color:rgb(255,0,7)">@'
+                                       b'<span style="color:rgb(0,0,0)">\n'
+                                       b'<span style="color:rgb(1,22,133)">\xc3\xa9')
+    bg = b"\x1b[48;2;25;25;25m"
+    assert oracle.encode(cells, 1) == (bg + b"\x1b[38;2;255;0;7m@\x1b[0m" +
+                                       bg + b"\x1b[38;2;0;0;0m\n\x1b[0m" +
+                                       bg + b"\x1b[38;2;1;22;133m\xc3\xa9\x1b[0m")
+    assert oracle.encode(cells[:0], 1, image=False) == b"\x1b[1;1H"
