@@ -76,9 +76,9 @@ __device__ __forceinline__ float xform_row(const float* M, int i, float x, float
 // empty.
 __device__ __forceinline__ int sat_index(float f)
 {
-    if (!(f > 0.0f)) return 0;
-    if (f >= 1073741824.0f) return 1 << 30;
-    return (int)f;
+    // fmaxf returns 0 for NaN (IEEE maxNum), and the clamp keeps the
+    // conversion in range
+    return (int)fminf(fmaxf(f, 0.0f), 1073741824.0f);
 }
 
 // Triangle::clone + mul + aabb + pixel domain + a (rasterizer.rs:56-67).

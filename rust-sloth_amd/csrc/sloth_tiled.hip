@@ -506,9 +506,23 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
     const int32_t* fnl = nl + fr.nl_off;
     uint32_t* fc = cells + fr.cell_off;
     float* fz = zbuf + fr.z_off;
-    // Stores the cells and depths of slot s from its winning cell (or blank).
-    auto store = [&](uint32_t s, uint64_t k, uint32_t cell, float z) {
-        const int row = g.tr0 + (int)(s >> sws), col = g.tc0 + 2 * (int)(s & ((1u << sws) - 1));
+    // A thread owns the slots of one column of the tile, every row_step-th
+    // row (the block is at least one tile row of slots wide).
+    const uint32_t my_sc = threadIdx.x & ((1u << sws) - 1);
+    const int my_col = g.tc0 + 2 * (int)my_sc;
+    const int row_step = (int)(blockDim.x >> sws);
+    const uint64_t id_step = (uint64_t)row_step * W;
+    // fn(slot, row, id of the slot's first cell) over this thread's slots
+    // inside the frame
+    auto for_my_slots = [&](auto&& fn) {
+        if (my_col >= g.tc1) return;
+        int row = g.tr0 + (int)(threadIdx.x >> sws);
+        uint64_t id = (uint64_t)row * W + (uint64_t)my_col;
+        for (; row < g.tr1; row += row_step, id += id_step)
+            fn(((uint32_t)(row - g.tr0) << sws) | my_sc, row, id);
+    };
+    // Stores the cells and depths of a slot from its winning cell (or blank).
+    auto store = [&](int row, uint64_t id, uint64_t k, uint32_t cell, float z) {
         uint32_t c0 = CELL_BLANK, c1 = CELL_BLANK;
         float z0 = F32_MAX;
         int32_t wt = -1;
@@ -517,7 +531,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             z0 = z;
             wt = (int32_t)((uint32_t)k >> 1);
         }
-        if (fr.image && col == 0) {
+        if (fr.image && my_col == 0) {
             // cell row*W + 1: the newline of row `row` comes after every pixel
             // of its triangle's row and before its next row, so it overwrites
             // a wrapped pixel (drawn in row-1) of the same or an earlier
@@ -525,7 +539,6 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             const int32_t nlr = fnl[row];
             if (nlr >= 0 && (k == KEY_EMPTY || nlr >= wt)) c1 = CELL_NEWLINE;
         }
-        const uint64_t id = (uint64_t)row * W + (uint64_t)col;
         *(uint2*)(fc + id) = make_uint2(c0, c1);
         *(float2*)(fz + id) = make_float2(z0, F32_MAX);
         if (win) {
@@ -534,15 +547,11 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             *(int2*)(win + fr.z_off + id) = make_int2(wv, -1);
         }
     };
-    auto slot_valid = [&](uint32_t s) {
-        return g.tr0 + (int)(s >> sws) < g.tr1 && g.tc0 + 2 * (int)(s & ((1u << sws) - 1)) < g.tc1;
-    };
 
     if (n == 0) {
         // no triangle reaches the tile: blank cells, f32::MAX depths, and the
         // newline cells of column 1
-        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x)
-            if (slot_valid(s)) store(s, KEY_EMPTY, CELL_BLANK, F32_MAX);
+        for_my_slots([&](uint32_t, int row, uint64_t id) { store(row, id, KEY_EMPTY, CELL_BLANK, F32_MAX); });
     } else {
         auto ramp_of = [&](uint32_t r) -> const Ramp& { return r < LDS_RAMPS ? s_ramp[r] : ramps[r]; };
         uint64_t* keys = smem;
@@ -554,7 +563,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
 
         // Winner of slot s with key k, re-evaluated from its cache row; false
         // when the key's triangle is not in the current chunk.
-        auto winner = [&](uint32_t s, uint64_t k, uint32_t& cell, float& z) -> bool {
+        auto winner = [&](int row, uint64_t k, uint32_t& cell, float& z) -> bool {
             const uint32_t T = (uint32_t)k >> 1, sub = (uint32_t)k & 1u;
             uint32_t h = hash_tri(T), tag;
             for (uint32_t probe = 0; probe < HASH_CAP; ++probe) {
@@ -565,7 +574,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             if (tag != T) return false;
             const float4* row4 = (const float4*)(cache + hrow[h] * CACHE_WORDS);
             const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
-            const int row = g.tr0 + (int)(s >> sws), col = g.tc0 + 2 * (int)(s & ((1u << sws) - 1));
+            const int col = my_col;
             const float px = (float)(sub ? (col >> 1) : ((col + g.W) >> 1));
             const float py = (float)(sub ? row : row - 1);
             // the same expressions as edges(), depth() and shade_value()
@@ -628,22 +637,21 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 // slots whose key now belongs to this chunk store that winner
                 // (a later chunk that changes the key stores again; the same
                 // thread owns slot s throughout, so its last store lands last)
-                for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+                for_my_slots([&](uint32_t s, int row, uint64_t id) {
                     const uint64_t k = keys[s];
                     uint32_t cell;
                     float z;
-                    if (k != KEY_EMPTY && winner(s, k, cell, z)) store(s, k, cell, z);
-                }
+                    if (k != KEY_EMPTY && winner(row, k, cell, z)) store(row, id, k, cell, z);
+                });
         }
-        for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
-            if (!slot_valid(s)) continue;
+        for_my_slots([&](uint32_t s, int row, uint64_t id) {
             const uint64_t k = keys[s];
-            if (multi && k != KEY_EMPTY) continue;   // stored after its chunk
+            if (multi && k != KEY_EMPTY) return;   // stored after its chunk
             uint32_t cell = CELL_BLANK;
             float z = F32_MAX;
-            if (k != KEY_EMPTY) winner(s, k, cell, z);
-            store(s, k, cell, z);
-        }
+            if (k != KEY_EMPTY) winner(row, k, cell, z);
+            store(row, id, k, cell, z);
+        });
     }
     // image mode: the H cells after the W*H grid are never written (context.rs:38-40)
     if (fr.image && tx == 0 && ty == 0)
