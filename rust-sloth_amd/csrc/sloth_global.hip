@@ -158,8 +158,9 @@ __global__ void __launch_bounds__(256) k_resolve_global(const Draw* __restrict__
             best_cell = cell;
             wt = win_raw ? (int32_t)(uint32_t)k : (int32_t)tri;   // raw: triangle << 1 | sub
         }
-        zbuf[fr.z_off + c] = z;
-        if (win) win[fr.z_off + c] = wt;
+        // streaming stores: outputs are written once
+        __builtin_nontemporal_store(z, zbuf + fr.z_off + c);
+        if (win) __builtin_nontemporal_store(wt, win + fr.z_off + c);
     }
     if (c >= 1 && c - 1 < WH) {
         uint64_t k = fk[c - 1];
@@ -187,7 +188,7 @@ __global__ void __launch_bounds__(256) k_resolve_global(const Draw* __restrict__
             }
         }
     }
-    cells[fr.cell_off + c] = best_cell;
+    __builtin_nontemporal_store(best_cell, cells + fr.cell_off + c);
 }
 
 }  // namespace sloth

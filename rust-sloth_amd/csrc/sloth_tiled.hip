@@ -539,12 +539,15 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             const int32_t nlr = fnl[row];
             if (nlr >= 0 && (k == KEY_EMPTY || nlr >= wt)) c1 = CELL_NEWLINE;
         }
-        *(uint2*)(fc + id) = make_uint2(c0, c1);
-        *(float2*)(fz + id) = make_float2(z0, F32_MAX);
+        // streaming stores: the frame is written once and not read back here
+        __builtin_nontemporal_store((uint64_t)c0 | ((uint64_t)c1 << 32), (uint64_t*)(fc + id));
+        __builtin_nontemporal_store((uint64_t)__float_as_uint(z0) | ((uint64_t)__float_as_uint(F32_MAX) << 32),
+                                    (uint64_t*)(fz + id));
         if (win) {
             // win_raw keeps the key's low word (triangle << 1 | sub) for the shading pass
             const int32_t wv = (k == KEY_EMPTY) ? -1 : (win_raw ? (int32_t)(uint32_t)k : wt);
-            *(int2*)(win + fr.z_off + id) = make_int2(wv, -1);
+            __builtin_nontemporal_store((uint64_t)(uint32_t)wv | ((uint64_t)0xFFFFFFFFu << 32),
+                                        (uint64_t*)(win + fr.z_off + id));
         }
     };
 
@@ -655,7 +658,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
     }
     // image mode: the H cells after the W*H grid are never written (context.rs:38-40)
     if (fr.image && tx == 0 && ty == 0)
-        for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) fc[(uint64_t)W * H + i] = CELL_BLANK;
+        for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) __builtin_nontemporal_store(CELL_BLANK, fc + (uint64_t)W * H + i);
 }
 
 // One workgroup per tile; grid (tile x, tile y, frame - f_lo).  Blocks past a
