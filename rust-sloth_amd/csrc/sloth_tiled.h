@@ -6,19 +6,24 @@
 // a tile edge, and a tile can resolve its cells alone.
 //
 //   k_tile_clear   tile cursors <- 0, newline rows <- -1, overflow <- 0,
-//                  image-mode tail cells <- blank
+//                  careful-list length <- 0
 //   k_bin          one thread per triangle: setup, classification, candidate
 //                  box, newline rows (LDS max per block, one atomicMax per
 //                  row), and the triangle index appended to every tile its box
-//                  reaches, with one atomicAdd per distinct tile per wavefront
-//                  (ballot + prefix popcount)
+//                  reaches: boxes within 2 x 2 tiles count into a 16 x 16 LDS
+//                  window of tiles, with one global atomicAdd per window tile
+//                  per block; others (and the careful list) append directly
+//   k_careful      one wavefront per careful triangle: exact box from row
+//                  spans, then its tile entries
 //   k_tile_raster  one 256-thread workgroup per tile, grid (tile x, tile y,
 //                  frame): each list entry's transformed triangle goes to an
 //                  LDS row (found by a hash of its index), its fragments do a
 //                  64-bit LDS atomicMin of the packed keys, then every slot
 //                  re-evaluates its winner from the LDS row and stores its two
-//                  cells and depths (coalesced).  Lists longer than the 256
-//                  LDS rows run in chunks, storing winners after each chunk.
+//                  cells and depths (coalesced, streaming stores).  Lists
+//                  longer than the 256 LDS rows run in chunks, storing winners
+//                  after each chunk.  The image-mode tail cells are written by
+//                  the frame's first tile.
 //
 // Tile lists live in one pool at fixed per-tile offsets, sized from an exact
 // counting run of k_bin plus headroom.  If a later run would exceed a tile's

@@ -594,7 +594,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
 
         for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
         for (uint32_t base = 0; base < n; base += CHUNK) {
-            if (base) __syncthreads();   // the previous chunk's staging no longer reads its rows
+            if (base) __syncthreads();   // the previous chunk's winner stores no longer read its rows
             for (uint32_t h = threadIdx.x; h < HASH_CAP; h += blockDim.x) htag[h] = EMPTY32;
             __syncthreads();
             const uint32_t p = threadIdx.x;
