@@ -176,8 +176,13 @@ std::string load_obj(const std::string& path, const std::string& data, std::vect
     std::string name = "unnamed";
     int mat = -1;
     std::vector<uint32_t> faces;
+    // a model's faces are resolved when it ends, so they may name vertices
+    // defined later inside the model
+    bool bad_index = false;
     auto end_model = [&]() {
         if (!faces.empty()) {
+            const uint32_t npos = (uint32_t)(pos.size() / 3);
+            for (uint32_t v : faces) bad_index |= v >= npos;
             models.push_back(Model{name, mat, faces});
             faces.clear();
         }
@@ -209,8 +214,8 @@ std::string load_obj(const std::string& path, const std::string& data, std::vect
                 const std::string v = w[i].substr(0, w[i].find('/'));
                 long long idx;
                 if (!parse_int(v, idx) || idx == 0) return "line " + std::to_string(ln) + ": bad face index";
-                idx = idx < 0 ? npos + idx : idx - 1;
-                if (idx < 0 || idx >= npos) return "line " + std::to_string(ln) + ": face index out of range";
+                idx = idx < 0 ? npos + idx : idx - 1;   // negative: relative to the vertices so far
+                if (idx < 0 || idx >= (1ll << 32) - 1) return "line " + std::to_string(ln) + ": face index out of range";
                 ids.push_back((uint32_t)idx);
             }
             // points and lines are ignored; polygons fan out from the first corner
@@ -235,6 +240,7 @@ std::string load_obj(const std::string& path, const std::string& data, std::vect
         }
     }
     end_model();
+    if (bad_index) return "face index beyond the vertices of its model";
     const bool has_vcol = !vcol.empty();
     if (has_vcol && vcol.size() != pos.size()) return "vertex colours on some positions only";
     for (const Model& md : models) {

@@ -65,10 +65,19 @@ def test_obj_errors(tmp_path):
     (tmp_path / "k.mtl").write_text("newmtl k\nKd 1 1 1\n")
     (tmp_path / "nomat.obj").write_text("mtllib k.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
     (tmp_path / "range.obj").write_text("v 0 0 0\nv 1 0 0\nf 1 2 3\n")
+    (tmp_path / "late.obj").write_text("v 0 0 0\nv 1 0 0\nf 1 2 3\no next\nv 0 1 0\nf 1 2 3\n")
     (tmp_path / "x.ply").write_text("")
-    for name in ("nolib.obj", "nomat.obj", "range.obj", "x.ply", "absent.obj"):
+    for name in ("nolib.obj", "nomat.obj", "range.obj", "late.obj", "x.ply", "absent.obj"):
         with pytest.raises(sloth_amd.SlothError):
             sloth_amd.ModelFile(str(tmp_path / name))
+
+
+def test_obj_face_before_its_vertex(tmp_path):
+    """Faces resolve when their model ends (tobj), so a face may name a vertex
+    defined after it inside the same model."""
+    (tmp_path / "f.obj").write_text("v 0 0 0\nv 1 0 0\nf 1 2 3\nv 0 1 0\n")
+    (pos, _, _, _), = _load(str(tmp_path / "f.obj"))
+    assert np.array_equal(pos[0], np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32))
 
 
 def test_cube_obj():
