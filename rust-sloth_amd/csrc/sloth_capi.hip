@@ -1251,8 +1251,10 @@ struct sloth_shading {
     uint64_t n_slots = 0;
     bool has_target = false;
     bool has_params = false;
-    DevBuf d_faces, d_vis, d_G, d_vcolor, d_vnormal, d_light, d_target, d_out, d_partial, d_tri_acc;
-    DevBuf d_adj_off, d_adj, d_dcolor, d_dnormal;
+    DevBuf d_faces, d_vis, d_G, d_vcolor, d_vnormal, d_light, d_target, d_out, d_partial, d_vert_acc;
+    DevBuf d_bwd_tile;            // backward tiles per frame (prefix, n_frames + 1)
+    uint64_t n_bwd_tiles = 0;
+    DevBuf d_dcolor, d_dnormal;
     std::vector<double> partial;
 };
 
@@ -1272,10 +1274,12 @@ static ShadeArgs shade_args(sloth_shading* sh, bool with_target, float* out)
     a.target = with_target ? sh->d_target.as<float>() : nullptr;
     a.out = out;
     a.partial = sh->d_partial.as<double>();
-    a.tri_acc = sh->d_tri_acc.as<double>();
+    a.vert_acc = sh->d_vert_acc.as<double>();
     a.n_slots = sh->n_slots;
     a.n_tris = sh->n_tris;
     a.n_verts = sh->n_verts;
+    a.bwd_tile_begin = sh->d_bwd_tile.as<uint64_t>();
+    a.n_bwd_tiles = sh->n_bwd_tiles;
     return a;
 }
 
@@ -1339,28 +1343,22 @@ int sloth_shading_create(sloth_batch* b, const uint32_t* faces, uint32_t n_tris,
     sh->n_tris = n_tris;
     sh->n_verts = n_verts;
     sh->n_slots = j.n_z;
-    // vertex -> (triangle, corner) adjacency, ascending
-    std::vector<uint32_t> adj_off(n_verts + 1, 0), adj((size_t)n_tris * 3);
-    for (uint64_t i = 0; i < (uint64_t)n_tris * 3; ++i) adj_off[faces[i] + 1]++;
-    for (uint32_t v = 0; v < n_verts; ++v) adj_off[v + 1] += adj_off[v];
-    {
-        std::vector<uint32_t> fill(adj_off.begin(), adj_off.end() - 1);
-        for (uint64_t i = 0; i < (uint64_t)n_tris * 3; ++i) adj[fill[faces[i]]++] = (uint32_t)i;
-    }
     std::vector<float> G(j.draws.size() * 9);
     for (size_t d = 0; d < j.draws.size(); ++d) normal_map(j.draws[d].M, &G[d * 9]);
     const uint32_t nb = shade_blocks(sh->n_slots);
     sh->partial.resize((size_t)nb * 4);
-    if ((rc = sh->d_faces.reserve(std::max<size_t>(4, (size_t)n_tris * 12))) ||
+    std::vector<uint64_t> tile_begin(j.frames.size() + 1);
+    bwd_tiles(j.frames.data(), (uint32_t)j.frames.size(), tile_begin.data());
+    sh->n_bwd_tiles = tile_begin.back();
+    if ((rc = sh->d_bwd_tile.reserve(tile_begin.size() * 8)) ||
+        (rc = sh->d_faces.reserve(std::max<size_t>(4, (size_t)n_tris * 12))) ||
         (rc = sh->d_vis.reserve(std::max<size_t>(4, sh->n_slots * 4))) ||
         (rc = sh->d_G.reserve(std::max<size_t>(4, G.size() * 4))) ||
         (rc = sh->d_vcolor.reserve(std::max<size_t>(4, (size_t)n_verts * 12))) ||
         (rc = sh->d_vnormal.reserve(std::max<size_t>(4, (size_t)n_verts * 12))) ||
         (rc = sh->d_light.reserve(16)) ||
         (rc = sh->d_partial.reserve(std::max<size_t>(8, (size_t)nb * 32))) ||
-        (rc = sh->d_tri_acc.reserve(std::max<size_t>(8, (size_t)n_tris * 18 * 8))) ||
-        (rc = sh->d_adj_off.reserve((size_t)(n_verts + 1) * 4)) ||
-        (rc = sh->d_adj.reserve(std::max<size_t>(4, adj.size() * 4))) ||
+        (rc = sh->d_vert_acc.reserve(std::max<size_t>(8, (size_t)n_verts * 6 * 8))) ||
         (rc = sh->d_dcolor.reserve(std::max<size_t>(4, (size_t)n_verts * 12))) ||
         (rc = sh->d_dnormal.reserve(std::max<size_t>(4, (size_t)n_verts * 12)))) {
         delete sh;
@@ -1369,8 +1367,7 @@ int sloth_shading_create(sloth_batch* b, const uint32_t* faces, uint32_t n_tris,
     hipError_t e = hipSuccess;
     if (n_tris) e = hipMemcpy(sh->d_faces.p, faces, (size_t)n_tris * 12, hipMemcpyHostToDevice);
     if (e == hipSuccess && !G.empty()) e = hipMemcpy(sh->d_G.p, G.data(), G.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(sh->d_adj_off.p, adj_off.data(), adj_off.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess && !adj.empty()) e = hipMemcpy(sh->d_adj.p, adj.data(), adj.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(sh->d_bwd_tile.p, tile_begin.data(), tile_begin.size() * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         delete sh;
         return fail(SLOTH_EHIP, "shading upload: %s", hipGetErrorString(e));
@@ -1462,9 +1459,7 @@ int sloth_shading_backward(sloth_shading* sh, float* d_vcolor, float* d_vnormal,
     hipStream_t s = sh->b->stream;
     ShadeArgs a = shade_args(sh, true, nullptr);
     if ((rc = launch_shade(a, true, s))) return rc;
-    if ((rc = launch_shade_gather(a, sh->d_adj_off.as<uint32_t>(), sh->d_adj.as<uint32_t>(), sh->d_dcolor.as<float>(),
-                                  sh->d_dnormal.as<float>(), s)))
-        return rc;
+    if ((rc = launch_shade_finish(a, sh->d_dcolor.as<float>(), sh->d_dnormal.as<float>(), s))) return rc;
     if (d_vcolor && sh->n_verts)
         HIP_TRY(hipMemcpyAsync(d_vcolor, sh->d_dcolor.p, (size_t)sh->n_verts * 12, hipMemcpyDeviceToHost, s));
     if (d_vnormal && sh->n_verts)
@@ -1489,8 +1484,7 @@ int sloth_shading_time(sloth_shading* sh, uint32_t steps, uint32_t warmup, float
         int r;
         if ((r = launch_shade(fwd, false, s))) return r;
         if ((r = launch_shade(bwd, true, s))) return r;
-        return launch_shade_gather(bwd, sh->d_adj_off.as<uint32_t>(), sh->d_adj.as<uint32_t>(),
-                                   sh->d_dcolor.as<float>(), sh->d_dnormal.as<float>(), s);
+        return launch_shade_finish(bwd, sh->d_dcolor.as<float>(), sh->d_dnormal.as<float>(), s);
     };
     for (uint32_t i = 0; i < warmup; ++i)
         if ((rc = step())) return rc;

@@ -22,14 +22,18 @@ struct ShadeArgs {
     const float* target;      // per z slot [3], or null (no loss)
     float* out;               // per z slot [3], or null
     double* partial;          // per block: loss, dL.x, dL.y, dL.z
-    double* tri_acc;          // [n_tris][3 corners][6]: d color, d normal
+    double* vert_acc;         // [n_verts][6]: d color, d normal (binary64 sums)
     uint64_t n_slots;
     uint32_t n_tris, n_verts;
+    const uint64_t* bwd_tile_begin;   // per frame (+1): first backward tile (64 x 4 z entries)
+    uint64_t n_bwd_tiles;
 };
 
 uint32_t shade_blocks(uint64_t n_slots);
+// Backward tiles (64 x 4 z entries) per frame, prefix-summed: begin[0..n_frames]
+void bwd_tiles(const Frame* frames, uint32_t n_frames, uint64_t* begin);
 int launch_shade(const ShadeArgs& a, bool backward, hipStream_t s);
-int launch_shade_gather(const ShadeArgs& a, const uint32_t* adj_off, const uint32_t* adj, float* d_color,
-                        float* d_normal, hipStream_t s);
+// binary64 vertex sums -> binary32 gradients
+int launch_shade_finish(const ShadeArgs& a, float* d_color, float* d_normal, hipStream_t s);
 
 }  // namespace sloth

@@ -16,8 +16,9 @@
 //
 // Backward: per fragment, with g = 2 (out - target) and dI = g . C,
 //   d c[v_k] += lambda_k I g,  d n[v_k] += dI lambda_k G^T L,  d L += dI n.
-// Per-(triangle, corner) sums are added in double with hardware atomics; a
-// gather kernel then sums each vertex's corners in a fixed (CSR) order.
+// Per-vertex sums are formed per tile of z entries in LDS and added to the
+// vertices' binary64 accumulators with hardware atomics, then converted to
+// binary32.
 #include "sloth_shade.h"
 
 namespace sloth {
@@ -34,51 +35,94 @@ __device__ __forceinline__ const Draw& shade_draw(const Draw* __restrict__ draws
     return draws[lo];
 }
 
-template <bool BWD>
-__global__ void __launch_bounds__(SHADE_BLOCK) k_shade(ShadeArgs p)
+// One visible fragment of z slot g (frame fr, visibility word v >= 0): the
+// barycentric weights, the interpolated normal (mapped by G) and colour, and
+// the output colour.
+struct Fragment {
+    float lam[3];
+    float n0, n1, n2, c0, c1, c2, I;
+    const float* G;
+    uint32_t vid[3];   // the triangle's vertices
+};
+
+__device__ __forceinline__ Fragment shade_fragment(const ShadeArgs& p, const Frame& fr, uint64_t g, int32_t v)
+{
+    Fragment f;
+    const uint32_t T = (uint32_t)v >> 1, sub = (uint32_t)v & 1u;
+    uint32_t px, py;
+    key_pixel((uint32_t)(g - fr.z_off), sub, fr.W, px, py);
+    const uint32_t d = (uint32_t)(&shade_draw(p.draws, fr, T) - p.draws);
+    const Draw& dr = p.draws[d];
+    const uint32_t li = T - dr.tri_begin;
+    Tri t;
+    setup_tri(dr.M, dr.xyz + (size_t)li * 9, fr.W, fr.H, t);
+    float w0, w1, w2;
+    edges(t, (float)px, (float)py, w0, w1, w2);
+    f.lam[0] = t.a * w0;
+    f.lam[1] = t.a * w1;
+    f.lam[2] = t.a * w2;
+    f.G = p.G + (size_t)d * 9;
+    const float* G = f.G;
+    f.n0 = f.n1 = f.n2 = f.c0 = f.c1 = f.c2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t vid = p.faces[(size_t)li * 3 + k];
+        f.vid[k] = vid;
+        const float* nk = p.vnormal + (size_t)vid * 3;
+        const float* ck = p.vcolor + (size_t)vid * 3;
+        const float a0 = nk[0], a1 = nk[1], a2 = nk[2];
+        f.n0 += f.lam[k] * (G[0] * a0 + G[1] * a1 + G[2] * a2);
+        f.n1 += f.lam[k] * (G[3] * a0 + G[4] * a1 + G[5] * a2);
+        f.n2 += f.lam[k] * (G[6] * a0 + G[7] * a1 + G[8] * a2);
+        f.c0 += f.lam[k] * ck[0];
+        f.c1 += f.lam[k] * ck[1];
+        f.c2 += f.lam[k] * ck[2];
+    }
+    f.I = p.light[0] * f.n0 + p.light[1] * f.n1 + p.light[2] * f.n2;
+    return f;
+}
+
+// Block sums of up to four doubles in a fixed order, written to partial[block * 4 + k].
+__device__ __forceinline__ void block_partials(double v0, double v1, double v2, double v3, double* partial,
+                                               uint32_t block)
 {
     __shared__ double s_red[4][SHADE_BLOCK / 64];
+    for (int sh = 32; sh >= 1; sh >>= 1) {
+        v0 += __shfl_xor(v0, sh);
+        v1 += __shfl_xor(v1, sh);
+        v2 += __shfl_xor(v2, sh);
+        v3 += __shfl_xor(v3, sh);
+    }
+    const int wv = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        s_red[0][wv] = v0;
+        s_red[1][wv] = v1;
+        s_red[2][wv] = v2;
+        s_red[3][wv] = v3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0.0;
+        for (int w = 0; w < SHADE_BLOCK / 64; ++w) s += s_red[threadIdx.x][w];
+        partial[(size_t)block * 4 + threadIdx.x] = s;
+    }
+}
+
+// Forward: one thread per z slot; output colours and per-block loss sums.
+__global__ void __launch_bounds__(SHADE_BLOCK) k_shade_fwd(ShadeArgs p)
+{
     const uint64_t g = (uint64_t)blockIdx.x * SHADE_BLOCK + threadIdx.x;
-    double loss = 0.0, dl0 = 0.0, dl1 = 0.0, dl2 = 0.0;
+    double loss = 0.0;
     if (g < p.n_slots) {
         uint32_t lo = 0, hi = p.n_frames - 1;
         while (lo < hi) {
             uint32_t mid = (lo + hi + 1) >> 1;
             if (p.frames[mid].z_off <= g) lo = mid; else hi = mid - 1;
         }
-        const Frame& fr = p.frames[lo];
         const int32_t v = p.vis[g];
         if (v >= 0) {
-            const uint32_t T = (uint32_t)v >> 1, sub = (uint32_t)v & 1u;
-            uint32_t px, py;
-            key_pixel((uint32_t)(g - fr.z_off), sub, fr.W, px, py);
-            const uint32_t d = (uint32_t)(&shade_draw(p.draws, fr, T) - p.draws);
-            const Draw& dr = p.draws[d];
-            const uint32_t li = T - dr.tri_begin;
-            Tri t;
-            setup_tri(dr.M, dr.xyz + (size_t)li * 9, fr.W, fr.H, t);
-            float w0, w1, w2;
-            edges(t, (float)px, (float)py, w0, w1, w2);
-            const float lam[3] = {t.a * w0, t.a * w1, t.a * w2};
-            const float* G = p.G + (size_t)d * 9;
-            const float L0 = p.light[0], L1 = p.light[1], L2 = p.light[2];
-            float n0 = 0.f, n1 = 0.f, n2 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
-            uint32_t vid[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                vid[k] = p.faces[(size_t)li * 3 + k];
-                const float* nk = p.vnormal + (size_t)vid[k] * 3;
-                const float* ck = p.vcolor + (size_t)vid[k] * 3;
-                const float a0 = nk[0], a1 = nk[1], a2 = nk[2];
-                n0 += lam[k] * (G[0] * a0 + G[1] * a1 + G[2] * a2);
-                n1 += lam[k] * (G[3] * a0 + G[4] * a1 + G[5] * a2);
-                n2 += lam[k] * (G[6] * a0 + G[7] * a1 + G[8] * a2);
-                c0 += lam[k] * ck[0];
-                c1 += lam[k] * ck[1];
-                c2 += lam[k] * ck[2];
-            }
-            const float I = L0 * n0 + L1 * n1 + L2 * n2;
-            const float o0 = I * c0, o1 = I * c1, o2 = I * c2;
+            const Fragment f = shade_fragment(p, p.frames[lo], g, v);
+            const float o0 = f.I * f.c0, o1 = f.I * f.c1, o2 = f.I * f.c2;
             if (p.out) {
                 p.out[g * 3 + 0] = o0;
                 p.out[g * 3 + 1] = o1;
@@ -89,51 +133,106 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade(ShadeArgs p)
                 const float r1 = o1 - p.target[g * 3 + 1];
                 const float r2 = o2 - p.target[g * 3 + 2];
                 loss = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
-                if (BWD) {
-                    const float g0 = 2.f * r0, g1 = 2.f * r1, g2 = 2.f * r2;
-                    const float dI = g0 * c0 + g1 * c1 + g2 * c2;
-                    // G^T L (model-space direction of the light for this view)
-                    const float gl0 = G[0] * L0 + G[3] * L1 + G[6] * L2;
-                    const float gl1 = G[1] * L0 + G[4] * L1 + G[7] * L2;
-                    const float gl2 = G[2] * L0 + G[5] * L1 + G[8] * L2;
-                    double* acc = p.tri_acc + (size_t)li * 18;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const float lk = lam[k];
-                        unsafeAtomicAdd(acc + k * 6 + 0, (double)(lk * I * g0));
-                        unsafeAtomicAdd(acc + k * 6 + 1, (double)(lk * I * g1));
-                        unsafeAtomicAdd(acc + k * 6 + 2, (double)(lk * I * g2));
-                        unsafeAtomicAdd(acc + k * 6 + 3, (double)(dI * lk * gl0));
-                        unsafeAtomicAdd(acc + k * 6 + 4, (double)(dI * lk * gl1));
-                        unsafeAtomicAdd(acc + k * 6 + 5, (double)(dI * lk * gl2));
-                    }
-                    dl0 = (double)(dI * n0);
-                    dl1 = (double)(dI * n1);
-                    dl2 = (double)(dI * n2);
-                }
             }
         }
     }
-    // block sums of the loss and the light gradient (fixed order)
-    for (int sh = 32; sh >= 1; sh >>= 1) {
-        loss += __shfl_xor(loss, sh);
-        dl0 += __shfl_xor(dl0, sh);
-        dl1 += __shfl_xor(dl1, sh);
-        dl2 += __shfl_xor(dl2, sh);
+    block_partials(loss, 0.0, 0.0, 0.0, p.partial, blockIdx.x);
+}
+
+#ifndef SLOTH_BWD_TILE_H
+#define SLOTH_BWD_TILE_H 16
+#endif
+constexpr uint32_t BWD_TILE_W = 64, BWD_TILE_H = SLOTH_BWD_TILE_H;   // z entries per backward tile
+constexpr uint32_t BWD_TABLE = 256;     // LDS rows for the tile's distinct vertices
+constexpr uint32_t BWD_PROBES = 32;     // a fragment that finds no row adds to HBM directly
+constexpr uint32_t BWD_EMPTY = 0xFFFFFFFFu;
+
+// Backward: a block takes 64 x BWD_TILE_H tiles of z entries (tile j of the
+// job; frames hold tile_begin[f] .. tile_begin[f+1]), grid-striding over the
+// tiles.  A fragment's 6 colour / normal terms for each of its three vertices
+// go into an LDS table keyed by vertex; after the tile, each table row adds
+// its sums to the vertex's binary64 accumulators with one global atomic per
+// term.  Several pixels of a tile share each vertex, so this takes most
+// same-address atomics off the memory system.  A vertex that finds no free
+// row adds its terms to the accumulators directly.  The light gradient is
+// summed per block.
+__global__ void __launch_bounds__(SHADE_BLOCK) k_shade_bwd(ShadeArgs p, const uint64_t* __restrict__ tile_begin,
+                                                           uint64_t n_tiles)
+{
+    __shared__ uint32_t s_key[BWD_TABLE];
+    __shared__ float s_acc[BWD_TABLE * 6];
+    double dl0 = 0.0, dl1 = 0.0, dl2 = 0.0;
+    for (uint64_t j = blockIdx.x; j < n_tiles; j += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < BWD_TABLE; i += SHADE_BLOCK) s_key[i] = BWD_EMPTY;
+        for (uint32_t i = threadIdx.x; i < BWD_TABLE * 6; i += SHADE_BLOCK) s_acc[i] = 0.0f;
+        __syncthreads();
+        uint32_t lo = 0, hi = p.n_frames - 1;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (tile_begin[mid] <= j) lo = mid; else hi = mid - 1;
+        }
+        const Frame& fr = p.frames[lo];
+        const uint32_t tiles_x = (fr.W + BWD_TILE_W - 1) / BWD_TILE_W;
+        const uint32_t local = (uint32_t)(j - tile_begin[lo]);
+        const uint32_t ty = local / tiles_x, tx = local - ty * tiles_x;
+        const uint32_t col = tx * BWD_TILE_W + (threadIdx.x % BWD_TILE_W);
+        for (uint32_t row = ty * BWD_TILE_H + (threadIdx.x / BWD_TILE_W); row < min(fr.H, (ty + 1) * BWD_TILE_H);
+             row += SHADE_BLOCK / BWD_TILE_W) {
+            if (col >= fr.W) break;
+            const uint64_t g = fr.z_off + (uint64_t)row * fr.W + col;
+            const int32_t v = p.vis[g];
+            if (v >= 0) {
+                const Fragment f = shade_fragment(p, fr, g, v);
+                const float o0 = f.I * f.c0, o1 = f.I * f.c1, o2 = f.I * f.c2;
+                const float g0 = 2.f * (o0 - p.target[g * 3 + 0]);
+                const float g1 = 2.f * (o1 - p.target[g * 3 + 1]);
+                const float g2 = 2.f * (o2 - p.target[g * 3 + 2]);
+                const float dI = g0 * f.c0 + g1 * f.c1 + g2 * f.c2;
+                const float* G = f.G;
+                const float L0 = p.light[0], L1 = p.light[1], L2 = p.light[2];
+                // G^T L (model-space direction of the light for this view)
+                const float gl0 = G[0] * L0 + G[3] * L1 + G[6] * L2;
+                const float gl1 = G[1] * L0 + G[4] * L1 + G[7] * L2;
+                const float gl2 = G[2] * L0 + G[5] * L1 + G[8] * L2;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float lk = f.lam[k];
+                    const float terms[6] = {lk * f.I * g0, lk * f.I * g1, lk * f.I * g2,
+                                            dI * lk * gl0, dI * lk * gl1, dI * lk * gl2};
+                    const uint32_t vid = f.vid[k];
+                    uint32_t h = (vid * 0x9E3779B1u) >> 24;   // BWD_TABLE = 256 rows
+                    bool found = false;
+                    for (uint32_t probe = 0; probe < BWD_PROBES; ++probe) {
+                        const uint32_t old = atomicCAS(&s_key[h], BWD_EMPTY, vid);
+                        if (old == BWD_EMPTY || old == vid) {
+                            found = true;
+                            break;
+                        }
+                        h = (h + 1) & (BWD_TABLE - 1);
+                    }
+                    if (found) {
+                        float* acc = s_acc + h * 6;
+                        for (int m = 0; m < 6; ++m) atomicAdd(acc + m, terms[m]);
+                    } else {
+                        double* dst = p.vert_acc + (size_t)vid * 6;
+                        for (int m = 0; m < 6; ++m) unsafeAtomicAdd(dst + m, (double)terms[m]);
+                    }
+                }
+                dl0 += (double)(dI * f.n0);
+                dl1 += (double)(dI * f.n1);
+                dl2 += (double)(dI * f.n2);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < BWD_TABLE; i += SHADE_BLOCK) {
+            const uint32_t vid = s_key[i];
+            if (vid == BWD_EMPTY) continue;
+            double* dst = p.vert_acc + (size_t)vid * 6;
+            for (int m = 0; m < 6; ++m) unsafeAtomicAdd(dst + m, (double)s_acc[i * 6 + m]);
+        }
+        __syncthreads();   // the table is cleared for the next tile
     }
-    const int wv = threadIdx.x >> 6;
-    if (__lane_id() == 0) {
-        s_red[0][wv] = loss;
-        s_red[1][wv] = dl0;
-        s_red[2][wv] = dl1;
-        s_red[3][wv] = dl2;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        double s = 0.0;
-        for (int w = 0; w < SHADE_BLOCK / 64; ++w) s += s_red[threadIdx.x][w];
-        p.partial[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
-    }
+    block_partials(0.0, dl0, dl1, dl2, p.partial, blockIdx.x);
 }
 
 __global__ void k_fill_f64(double* __restrict__ a, uint64_t n)
@@ -142,52 +241,56 @@ __global__ void k_fill_f64(double* __restrict__ a, uint64_t n)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) a[i] = 0.0;
 }
 
-// Per-vertex sums of the (triangle, corner) accumulators, in CSR order.
-__global__ void k_shade_gather(const double* __restrict__ tri_acc, const uint32_t* __restrict__ adj_off,
-                               const uint32_t* __restrict__ adj, uint32_t n_verts, float* __restrict__ d_color,
+// The vertices' binary64 sums as the binary32 gradients.
+__global__ void k_shade_finish(const double* __restrict__ vert_acc, uint32_t n_verts, float* __restrict__ d_color,
                                float* __restrict__ d_normal)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n_verts) return;
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t e = adj_off[v]; e < adj_off[v + 1]; ++e) {
-        const double* a = tri_acc + (size_t)adj[e] * 6;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) s[j] += a[j];
-    }
-    d_color[v * 3 + 0] = (float)s[0];
-    d_color[v * 3 + 1] = (float)s[1];
-    d_color[v * 3 + 2] = (float)s[2];
-    d_normal[v * 3 + 0] = (float)s[3];
-    d_normal[v * 3 + 1] = (float)s[4];
-    d_normal[v * 3 + 2] = (float)s[5];
+    const double* a = vert_acc + (size_t)v * 6;
+    d_color[v * 3 + 0] = (float)a[0];
+    d_color[v * 3 + 1] = (float)a[1];
+    d_color[v * 3 + 2] = (float)a[2];
+    d_normal[v * 3 + 0] = (float)a[3];
+    d_normal[v * 3 + 1] = (float)a[4];
+    d_normal[v * 3 + 2] = (float)a[5];
 }
 
 uint32_t shade_blocks(uint64_t n_slots) { return (uint32_t)((n_slots + SHADE_BLOCK - 1) / SHADE_BLOCK); }
+
+void bwd_tiles(const Frame* frames, uint32_t n_frames, uint64_t* begin)
+{
+    begin[0] = 0;
+    for (uint32_t f = 0; f < n_frames; ++f) {
+        const uint64_t tx = (frames[f].W + BWD_TILE_W - 1) / BWD_TILE_W, ty = (frames[f].H + BWD_TILE_H - 1) / BWD_TILE_H;
+        begin[f + 1] = begin[f] + tx * ty;
+    }
+}
 
 int launch_shade(const ShadeArgs& a, bool backward, hipStream_t s)
 {
     const uint32_t nb = shade_blocks(a.n_slots);
     if (nb == 0) return SLOTH_OK;
     if (backward) {
-        const uint64_t n_acc = (uint64_t)a.n_tris * 18;
+        const uint64_t n_acc = (uint64_t)a.n_verts * 6;
         hipLaunchKernelGGL(k_fill_f64, dim3(std::max(1u, std::min(4096u, (uint32_t)((n_acc + 255) / 256)))),
-                           dim3(256), 0, s, a.tri_acc, n_acc);
+                           dim3(256), 0, s, a.vert_acc, n_acc);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_shade<true>, dim3(nb), dim3(SHADE_BLOCK), 0, s, a);
+        // nb blocks (the partial sums' count) stride over the backward tiles
+        if (a.n_bwd_tiles) hipLaunchKernelGGL(k_shade_bwd, dim3(nb), dim3(SHADE_BLOCK), 0, s, a, a.bwd_tile_begin,
+                                              a.n_bwd_tiles);
     } else {
-        hipLaunchKernelGGL(k_shade<false>, dim3(nb), dim3(SHADE_BLOCK), 0, s, a);
+        hipLaunchKernelGGL(k_shade_fwd, dim3(nb), dim3(SHADE_BLOCK), 0, s, a);
     }
     HIP_TRY(hipGetLastError());
     return SLOTH_OK;
 }
 
-int launch_shade_gather(const ShadeArgs& a, const uint32_t* adj_off, const uint32_t* adj, float* d_color,
-                        float* d_normal, hipStream_t s)
+int launch_shade_finish(const ShadeArgs& a, float* d_color, float* d_normal, hipStream_t s)
 {
     if (a.n_verts == 0) return SLOTH_OK;
-    hipLaunchKernelGGL(k_shade_gather, dim3((a.n_verts + 255) / 256), dim3(256), 0, s, a.tri_acc, adj_off, adj,
-                       a.n_verts, d_color, d_normal);
+    hipLaunchKernelGGL(k_shade_finish, dim3((a.n_verts + 255) / 256), dim3(256), 0, s, a.vert_acc, a.n_verts, d_color,
+                       d_normal);
     HIP_TRY(hipGetLastError());
     return SLOTH_OK;
 }
