@@ -283,8 +283,15 @@ int sloth_shading_forward(sloth_shading *s, const float *vcolor, const float *vn
 /* Backward of the loss at the parameters of the last forward call: gradients
  * float[n_verts][3], float[n_verts][3], float[3] (any may be NULL). */
 int sloth_shading_backward(sloth_shading *s, float *d_vcolor, float *d_vnormal, float *d_light);
-/* Times `steps` forward+backward passes on the device-resident parameters
- * and target after `warmup` untimed ones; *ms = elapsed milliseconds. */
+/* One training step in a single pass over the fragments: the loss (as
+ * sloth_shading_forward gives it) and its gradients (as
+ * sloth_shading_backward gives them) at vcolor, vnormal, light.  Needs a
+ * target; any output may be NULL. */
+int sloth_shading_step(sloth_shading *s, const float *vcolor, const float *vnormal, const float *light,
+                       double *loss, float *d_vcolor, float *d_vnormal, float *d_light);
+/* Times `steps` training steps (loss and gradients, as sloth_shading_step) on
+ * the device-resident parameters and target after `warmup` untimed ones;
+ * *ms = elapsed milliseconds. */
 int sloth_shading_time(sloth_shading *s, uint32_t steps, uint32_t warmup, float *ms);
 int sloth_shading_destroy(sloth_shading *s);
 

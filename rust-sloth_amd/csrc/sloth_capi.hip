@@ -1424,12 +1424,8 @@ static int shading_reduce(sloth_shading* sh, double* loss, double* dlight)
     return SLOTH_OK;
 }
 
-int sloth_shading_forward(sloth_shading* sh, const float* vcolor, const float* vnormal, const float* light,
-                          float* out_rgb, double* loss)
+static int shading_params(sloth_shading* sh, const float* vcolor, const float* vnormal, const float* light)
 {
-    ARG_CHECK(sh && vcolor && vnormal && light, "null argument");
-    int rc = ensure_device();
-    if (rc) return rc;
     hipStream_t s = sh->b->stream;
     if (sh->n_verts) {
         HIP_TRY(hipMemcpyAsync(sh->d_vcolor.p, vcolor, (size_t)sh->n_verts * 12, hipMemcpyHostToDevice, s));
@@ -1437,6 +1433,45 @@ int sloth_shading_forward(sloth_shading* sh, const float* vcolor, const float* v
     }
     HIP_TRY(hipMemcpyAsync(sh->d_light.p, light, 12, hipMemcpyHostToDevice, s));
     sh->has_params = true;
+    return SLOTH_OK;
+}
+
+// Backward launches (accumulator clear, per-tile backward with the loss,
+// conversion of the vertex sums) on the shading's stream.
+static int shading_backward_launch(sloth_shading* sh)
+{
+    hipStream_t s = sh->b->stream;
+    const ShadeArgs a = shade_args(sh, true, nullptr);
+    int rc = launch_shade(a, true, s);
+    if (rc) return rc;
+    return launch_shade_finish(a, sh->d_dcolor.as<float>(), sh->d_dnormal.as<float>(), s);
+}
+
+// Copies the gradients out and sums the per-block loss and light partials.
+static int shading_gradients(sloth_shading* sh, double* loss, float* d_vcolor, float* d_vnormal, float* d_light)
+{
+    hipStream_t s = sh->b->stream;
+    if (d_vcolor && sh->n_verts)
+        HIP_TRY(hipMemcpyAsync(d_vcolor, sh->d_dcolor.p, (size_t)sh->n_verts * 12, hipMemcpyDeviceToHost, s));
+    if (d_vnormal && sh->n_verts)
+        HIP_TRY(hipMemcpyAsync(d_vnormal, sh->d_dnormal.p, (size_t)sh->n_verts * 12, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double dl[3];
+    int rc = shading_reduce(sh, loss, dl);
+    if (rc) return rc;
+    if (d_light)
+        for (int k = 0; k < 3; ++k) d_light[k] = (float)dl[k];
+    return SLOTH_OK;
+}
+
+int sloth_shading_forward(sloth_shading* sh, const float* vcolor, const float* vnormal, const float* light,
+                          float* out_rgb, double* loss)
+{
+    ARG_CHECK(sh && vcolor && vnormal && light, "null argument");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = sh->b->stream;
+    if ((rc = shading_params(sh, vcolor, vnormal, light))) return rc;
     float* d_out = nullptr;
     if (out_rgb) {
         if ((rc = sh->d_out.reserve(std::max<size_t>(4, sh->n_slots * 12)))) return rc;
@@ -1456,20 +1491,20 @@ int sloth_shading_backward(sloth_shading* sh, float* d_vcolor, float* d_vnormal,
     if (!sh->has_target) return fail(SLOTH_ESTATE, "backward needs a target (sloth_shading_set_target)");
     int rc = ensure_device();
     if (rc) return rc;
-    hipStream_t s = sh->b->stream;
-    ShadeArgs a = shade_args(sh, true, nullptr);
-    if ((rc = launch_shade(a, true, s))) return rc;
-    if ((rc = launch_shade_finish(a, sh->d_dcolor.as<float>(), sh->d_dnormal.as<float>(), s))) return rc;
-    if (d_vcolor && sh->n_verts)
-        HIP_TRY(hipMemcpyAsync(d_vcolor, sh->d_dcolor.p, (size_t)sh->n_verts * 12, hipMemcpyDeviceToHost, s));
-    if (d_vnormal && sh->n_verts)
-        HIP_TRY(hipMemcpyAsync(d_vnormal, sh->d_dnormal.p, (size_t)sh->n_verts * 12, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    double dl[3];
-    if ((rc = shading_reduce(sh, nullptr, dl))) return rc;
-    if (d_light)
-        for (int k = 0; k < 3; ++k) d_light[k] = (float)dl[k];
-    return SLOTH_OK;
+    if ((rc = shading_backward_launch(sh))) return rc;
+    return shading_gradients(sh, nullptr, d_vcolor, d_vnormal, d_light);
+}
+
+int sloth_shading_step(sloth_shading* sh, const float* vcolor, const float* vnormal, const float* light,
+                       double* loss, float* d_vcolor, float* d_vnormal, float* d_light)
+{
+    ARG_CHECK(sh && vcolor && vnormal && light, "null argument");
+    if (!sh->has_target) return fail(SLOTH_ESTATE, "a step needs a target (sloth_shading_set_target)");
+    int rc = ensure_device();
+    if (rc) return rc;
+    if ((rc = shading_params(sh, vcolor, vnormal, light))) return rc;
+    if ((rc = shading_backward_launch(sh))) return rc;
+    return shading_gradients(sh, loss, d_vcolor, d_vnormal, d_light);
 }
 
 int sloth_shading_time(sloth_shading* sh, uint32_t steps, uint32_t warmup, float* ms)
@@ -1479,13 +1514,7 @@ int sloth_shading_time(sloth_shading* sh, uint32_t steps, uint32_t warmup, float
     int rc = ensure_device();
     if (rc) return rc;
     hipStream_t s = sh->b->stream;
-    ShadeArgs fwd = shade_args(sh, true, nullptr), bwd = fwd;
-    auto step = [&]() -> int {
-        int r;
-        if ((r = launch_shade(fwd, false, s))) return r;
-        if ((r = launch_shade(bwd, true, s))) return r;
-        return launch_shade_finish(bwd, sh->d_dcolor.as<float>(), sh->d_dnormal.as<float>(), s);
-    };
+    auto step = [&]() -> int { return shading_backward_launch(sh); };   // loss and gradients in one pass
     for (uint32_t i = 0; i < warmup; ++i)
         if ((rc = step())) return rc;
     HIP_TRY(hipStreamSynchronize(s));

@@ -154,14 +154,14 @@ constexpr uint32_t BWD_EMPTY = 0xFFFFFFFFu;
 // its sums to the vertex's binary64 accumulators with one global atomic per
 // term.  Several pixels of a tile share each vertex, so this takes most
 // same-address atomics off the memory system.  A vertex that finds no free
-// row adds its terms to the accumulators directly.  The light gradient is
-// summed per block.
+// row adds its terms to the accumulators directly.  The loss and the light
+// gradient are summed per block.
 __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_bwd(ShadeArgs p, const uint64_t* __restrict__ tile_begin,
                                                            uint64_t n_tiles)
 {
     __shared__ uint32_t s_key[BWD_TABLE];
     __shared__ float s_acc[BWD_TABLE * 6];
-    double dl0 = 0.0, dl1 = 0.0, dl2 = 0.0;
+    double loss = 0.0, dl0 = 0.0, dl1 = 0.0, dl2 = 0.0;
     for (uint64_t j = blockIdx.x; j < n_tiles; j += gridDim.x) {
         for (uint32_t i = threadIdx.x; i < BWD_TABLE; i += SHADE_BLOCK) s_key[i] = BWD_EMPTY;
         for (uint32_t i = threadIdx.x; i < BWD_TABLE * 6; i += SHADE_BLOCK) s_acc[i] = 0.0f;
@@ -184,9 +184,11 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_bwd(ShadeArgs p, const ui
             if (v >= 0) {
                 const Fragment f = shade_fragment(p, fr, g, v);
                 const float o0 = f.I * f.c0, o1 = f.I * f.c1, o2 = f.I * f.c2;
-                const float g0 = 2.f * (o0 - p.target[g * 3 + 0]);
-                const float g1 = 2.f * (o1 - p.target[g * 3 + 1]);
-                const float g2 = 2.f * (o2 - p.target[g * 3 + 2]);
+                const float r0 = o0 - p.target[g * 3 + 0];
+                const float r1 = o1 - p.target[g * 3 + 1];
+                const float r2 = o2 - p.target[g * 3 + 2];
+                loss += (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;   // as the forward pass sums it
+                const float g0 = 2.f * r0, g1 = 2.f * r1, g2 = 2.f * r2;
                 const float dI = g0 * f.c0 + g1 * f.c1 + g2 * f.c2;
                 const float* G = f.G;
                 const float L0 = p.light[0], L1 = p.light[1], L2 = p.light[2];
@@ -232,7 +234,7 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_bwd(ShadeArgs p, const ui
         }
         __syncthreads();   // the table is cleared for the next tile
     }
-    block_partials(0.0, dl0, dl1, dl2, p.partial, blockIdx.x);
+    block_partials(loss, dl0, dl1, dl2, p.partial, blockIdx.x);
 }
 
 __global__ void k_fill_f64(double* __restrict__ a, uint64_t n)

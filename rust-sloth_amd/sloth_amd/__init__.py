@@ -116,6 +116,7 @@ SIGNATURES = {
     "sloth_shading_set_target": (C.c_int, [_vp, _fp]),
     "sloth_shading_forward": (C.c_int, [_vp, _fp, _fp, _fp, _fp, C.POINTER(C.c_double)]),
     "sloth_shading_backward": (C.c_int, [_vp, _fp, _fp, _fp]),
+    "sloth_shading_step": (C.c_int, [_vp, _fp, _fp, _fp, C.POINTER(C.c_double), _fp, _fp, _fp]),
     "sloth_shading_time": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _fp]),
     "sloth_shading_destroy": (C.c_int, [_vp]),
     "sloth_screen_transform": (C.c_int, [C.c_uint32, C.c_uint32, _fp, C.c_uint32, _fp]),
@@ -578,7 +579,21 @@ class Shading:
         _check(load_library().sloth_shading_backward(self._h, _fptr(dc), _fptr(dn), _fptr(dl)))
         return dc, dn, dl
 
+    def step(self, vcolor, vnormal, light):
+        """Loss and gradients in one pass: (loss, d_vcolor, d_vnormal, d_light)."""
+        vc = np.ascontiguousarray(vcolor, np.float32).reshape(self.n_verts, 3)
+        vn = np.ascontiguousarray(vnormal, np.float32).reshape(self.n_verts, 3)
+        li = np.ascontiguousarray(light, np.float32).reshape(3)
+        loss = C.c_double()
+        dc = np.empty((self.n_verts, 3), np.float32)
+        dn = np.empty((self.n_verts, 3), np.float32)
+        dl = np.empty(3, np.float32)
+        _check(load_library().sloth_shading_step(self._h, _fptr(vc), _fptr(vn), _fptr(li), C.byref(loss),
+                                                 _fptr(dc), _fptr(dn), _fptr(dl)))
+        return float(loss.value), dc, dn, dl
+
     def time(self, steps: int, warmup: int) -> float:
+        """Milliseconds for `steps` training steps (loss and gradients)."""
         ms = C.c_float()
         _check(load_library().sloth_shading_time(self._h, steps, warmup, C.byref(ms)))
         return float(ms.value)

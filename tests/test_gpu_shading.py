@@ -98,6 +98,12 @@ def test_gradients_match_fp64_adjoint(sloth, views, W, H):
         err = np.linalg.norm(g.astype(np.float64) - ref) / np.linalg.norm(ref)
         say(f"  d{name}: norm-relative error vs FP64 adjoint {err:.2e}")
         assert err <= REL, (name, err)
+    # the one-pass training step gives the same loss and gradients
+    loss_s, dc_s, dn_s, dl_s = sh.step(vc, vn, L)
+    assert abs(loss_s - loss_cpu) <= REL * abs(loss_cpu)
+    for name, g, ref in (("color", dc_s, gc), ("normal", dn_s, gn), ("light", dl_s, gl)):
+        err = np.linalg.norm(g.astype(np.float64) - ref) / np.linalg.norm(ref)
+        assert err <= REL, ("step", name, err)
 
     # the FP64 adjoint against central differences, h = 1e-3, in float64
     h = 1e-3
