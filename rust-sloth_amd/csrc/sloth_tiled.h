@@ -23,7 +23,9 @@
 //                  cells and depths (coalesced, streaming stores).  Lists
 //                  longer than the 256 LDS rows run in chunks, storing winners
 //                  after each chunk.  The image-mode tail cells are written by
-//                  the frame's first tile.
+//                  the frame's first tile.  When many boxes are big, a variant
+//                  has each wavefront walk its lanes' big boxes with all 64
+//                  lanes.
 //
 // Tile lists live in one pool at fixed per-tile offsets, sized from an exact
 // counting run of k_bin plus headroom.  If a later run would exceed a tile's
@@ -54,6 +56,10 @@ struct TiledPlan {
     // in, (triangle, box) out; capacity from the counting run plus headroom
     DevBuf d_careful_cnt, d_careful_in, d_careful_box;
     uint32_t careful_cap = 0;
+    // set by size(): the lists hold enough big boxes (more than 128 pixels,
+    // one triangle in 512 or more) for the raster variant whose wavefronts
+    // walk those boxes together; SLOTH_BIG_WALK=0/1 overrides
+    bool big_walk = false;
 
     // Sets the tile grid of every frame and allocates per-tile and per-row
     // state.  Pool offsets become invalid until size() runs.

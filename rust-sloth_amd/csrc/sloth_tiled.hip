@@ -1,6 +1,7 @@
 // Tiled raster path (even widths): bin-to-tile and LDS tile raster.
 // The data flow is described in sloth_tiled.h.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "sloth_tiled.h"
@@ -9,6 +10,12 @@ namespace sloth {
 
 constexpr int BIN_BLOCK = 256;
 constexpr int RASTER_BLOCK = 256;
+
+// k_tile_raster<true> walks a box with more than BIG_BOX pixels in a tile by
+// the whole wavefront.  careful_cnt[BIG_CNT] is where the counting run of
+// k_bin counts the triangles whose (unclipped) box has more.
+constexpr uint32_t BIG_BOX = 128;
+constexpr uint32_t BIG_CNT = 2 + MAX_GROUPS;
 
 // ---------------------------------------------------------------------------
 // k_tile_clear
@@ -22,7 +29,7 @@ __global__ void k_tile_clear(uint32_t* __restrict__ cursor, uint64_t n_tiles, in
     for (uint64_t i = i0; i < n_tiles; i += stride) cursor[i] = 0;
     for (uint64_t i = i0; i < n_rows; i += stride) nl[i] = -1;
     for (uint64_t i = i0; i < n_frames; i += stride) overflow[i] = 0;
-    if (i0 == 0) careful_cnt[0] = careful_cnt[1] = 0;   // list length, first group's start
+    if (i0 == 0) careful_cnt[0] = careful_cnt[1] = careful_cnt[BIG_CNT] = 0;   // list length, first group's start
 }
 
 // ---------------------------------------------------------------------------
@@ -37,6 +44,7 @@ __global__ void k_tile_clear(uint32_t* __restrict__ cursor, uint64_t n_tiles, in
 constexpr uint32_t CAREFUL_FLAG = 0x80000000u;
 constexpr uint32_t LOOSE_FLAG = 0x40000000u;
 constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
+
 
 // Tile rectangles (inclusive, in tile units) that a candidate box reaches:
 // the direct part (2x < W, cell (2x, y)) and the wrapped part (2x >= W, cell
@@ -201,7 +209,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     const uint32_t tiles_x = fr.tiles_x;
 
     int y0 = 0, y1d = 0;
-    bool careful = false;
+    bool careful = false, big = false;
     uint32_t entry = tri;
     TileRects r;
     empty_rects(r);
@@ -216,6 +224,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
             box_x(t, cls == CLS_TIGHT, bx0, bx1);
             if (cls == CLS_BOX) entry |= LOOSE_FLAG;
             if (bx0 <= bx1) tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);
+            if (COUNT) big = (int64_t)(bx1 - bx0 + 1) * (int64_t)(t.y1d - t.y0) > (int64_t)BIG_BOX;
         } else if (cls != CLS_EMPTY) {
             careful = true;
         }
@@ -232,6 +241,10 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     }
     __syncthreads();
 
+    if (COUNT) {
+        const uint64_t m = __ballot(big);
+        if (__lane_id() == 0 && m) atomicAdd(&careful_cnt[BIG_CNT], (uint32_t)__popcll(m));
+    }
     if (careful) {
         const uint32_t idx = atomicAdd(careful_cnt, 1u);
         if (idx < careful_cap) careful_in[idx] = make_uint2(d, local);
@@ -437,22 +450,38 @@ struct TileGeom {
 };
 
 // Calls f(x, y, slot, sub) for every logical pixel of the box that lands in
-// the tile: the direct region (2x in [tc0,tc1), row y) and the wrapped region
-// (2x - W in [tc0,tc1), row y + 1).
+// the tile's wrapped region (2x - W in [tc0,tc1), row y + 1).
+template <class Fn>
+__device__ __forceinline__ void walk_wrapped(const TileGeom& g, int bx0, int bx1, int by0, int by1, Fn&& f)
+{
+    const int wx0 = (g.tc0 + g.W) >> 1;
+    const int xa = max(bx0, wx0), xb = min(bx1, ((g.tc1 + g.W) >> 1) - 1);
+    const int ya = max(by0, g.tr0 - 1), yb = min(by1, g.tr1 - 2);
+    for (int y = ya; y <= yb; ++y)
+        for (int x = xa; x <= xb; ++x) f(x, y, ((y + 1 - g.tr0) << g.sws) + (x - wx0), 0u);
+}
+
+// Calls f(x, y, slot, sub) for every logical pixel of the box that lands in
+// the tile: the direct region (2x in [tc0,tc1), row y), then the wrapped one.
 template <class Fn>
 __device__ __forceinline__ void walk_tile(const TileGeom& g, int bx0, int bx1, int by0, int by1, Fn&& f)
 {
-    int xa = max(bx0, g.tc0 >> 1), xb = min(bx1, (g.tc1 >> 1) - 1);
-    int ya = max(by0, g.tr0), yb = min(by1, g.tr1 - 1);
+    const int xa = max(bx0, g.tc0 >> 1), xb = min(bx1, (g.tc1 >> 1) - 1);
+    const int ya = max(by0, g.tr0), yb = min(by1, g.tr1 - 1);
     for (int y = ya; y <= yb; ++y)
         for (int x = xa; x <= xb; ++x) f(x, y, ((y - g.tr0) << g.sws) + (x - (g.tc0 >> 1)), 1u);
-    int wx0 = (g.tc0 + g.W) >> 1;
-    xa = max(bx0, wx0);
-    xb = min(bx1, ((g.tc1 + g.W) >> 1) - 1);
-    ya = max(by0, g.tr0 - 1);
-    yb = min(by1, g.tr1 - 2);
-    for (int y = ya; y <= yb; ++y)
-        for (int x = xa; x <= xb; ++x) f(x, y, ((y + 1 - g.tr0) << g.sws) + (x - wx0), 0u);
+    walk_wrapped(g, bx0, bx1, by0, by1, f);
+}
+
+// The direct part of the walk: the box clipped to the tile's logical pixels,
+// as tile-relative [xa, xb] x [ya, yb] (empty when xa > xb or ya > yb).
+__device__ __forceinline__ void direct_rect(const TileGeom& g, int bx0, int bx1, int by0, int by1, int& xa, int& xb,
+                                            int& ya, int& yb)
+{
+    xa = max(bx0, g.tc0 >> 1) - (g.tc0 >> 1);
+    xb = min(bx1, (g.tc1 >> 1) - 1) - (g.tc0 >> 1);
+    ya = max(by0, g.tr0) - g.tr0;
+    yb = min(by1, g.tr1 - 1) - g.tr0;
 }
 
 // Triangle index of a pool entry; the candidate box of a careful entry comes
@@ -482,7 +511,14 @@ __device__ __forceinline__ uint32_t entry_tri(uint32_t e, const uint4* __restric
 // key belongs to one of the chunk's triangles store that winner (the last
 // store of a slot is its final winner's, since a later winner comes from a
 // later chunk), and the empty slots are stored at the end.
+//
+// BIG: an entry whose box covers more than BIG_BOX of the tile's pixels in
+// its direct region is not walked by its own lane; after its own walk, each
+// wavefront takes its lanes' big boxes one at a time and all 64 lanes stride
+// over the box.  The variant costs registers on every path, so it is launched
+// only for lists with many big boxes (TiledPlan::big_walk).
 // Tile (tx, ty) of frame fr, whose list is pool[o, o + n).
+template <bool BIG>
 __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, const Ramp* __restrict__ ramps,
                                             const Ramp* s_ramp, const Frame& fr, uint32_t tx, uint32_t ty, uint64_t o,
                                             uint32_t n, int tws, int ths, const uint32_t* __restrict__ pool,
@@ -598,6 +634,8 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             for (uint32_t h = threadIdx.x; h < HASH_CAP; h += blockDim.x) htag[h] = EMPTY32;
             __syncthreads();
             const uint32_t p = threadIdx.x;
+            bool big = false;              // BIG: this lane's box is walked by its wavefront
+            uint32_t big_T = 0, big_rect = 0;
             if (base + p < n) {
                 int bx0, bx1, by0, by1;
                 bool given, tight;
@@ -626,14 +664,70 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                     by0 = t.y0;
                     by1 = t.y1d - 1;
                 }
-                if (bx0 <= bx1 && by0 <= by1)
-                    walk_tile(g, bx0, bx1, by0, by1, [&](int x, int y, int slot, uint32_t sub) {
+                if (bx0 <= bx1 && by0 <= by1) {
+                    auto test = [&](int x, int y, int slot, uint32_t sub) {
                         float w0, w1, w2;
                         edges(t, (float)x, (float)y, w0, w1, w2);
                         const float z = depth(t, w1, w2);
                         if (inside(w0, w1, w2) && z < F32_MAX)
                             atomicMin((unsigned long long*)&keys[slot], (unsigned long long)make_key(z, T, sub));
-                    });
+                    };
+                    if constexpr (BIG) {
+                        int xa, xb, ya, yb;
+                        direct_rect(g, bx0, bx1, by0, by1, xa, xb, ya, yb);
+                        if (xa <= xb && ya <= yb && (xb - xa + 1) * (yb - ya + 1) > (int)BIG_BOX) {
+                            big = true;
+                            big_T = T;
+                            big_rect =
+                                (uint32_t)xa | ((uint32_t)xb << 8) | ((uint32_t)ya << 16) | ((uint32_t)yb << 24);
+                        } else {
+                            for (int y = ya; y <= yb; ++y)
+                                for (int x = xa; x <= xb; ++x)
+                                    test(x + (g.tc0 >> 1), y + g.tr0, (y << sws) + x, 1u);
+                        }
+                        walk_wrapped(g, bx0, bx1, by0, by1, test);
+                    } else {
+                        walk_tile(g, bx0, bx1, by0, by1, test);
+                    }
+                }
+            }
+            if constexpr (BIG) {
+                // The wavefront's big boxes, one at a time: lane l of the box
+                // takes pixels i = l, l + 64, ... (row i / nx, column i % nx),
+                // with the expressions of edges() and depth() on the box's
+                // cache row (row p of the lane that holds it, written by this
+                // wavefront above).
+                uint64_t m = __ballot(big);
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const uint32_t T = __shfl(big_T, l), rc = __shfl(big_rect, l);
+                    const float4* row4 = (const float4*)(cache + ((threadIdx.x & ~63u) + (uint32_t)l) * CACHE_WORDS);
+                    const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
+                    const float x1 = q0.x, y1 = q0.y, x2 = q0.z, y2 = q0.w, x3 = q1.x, y3 = q1.y;
+                    const float A0 = x3 - x2, B0 = y3 - y2, A1 = x1 - x3, B1 = y1 - y3, A2 = x2 - x1, B2 = y2 - y1;
+                    const float z1 = q1.z, dz2 = q1.w, dz3 = q2.x, a = q2.y;
+                    const int xa = (int)(rc & 0xFFu), nx = (int)((rc >> 8) & 0xFFu) - xa + 1;
+                    const int ya = (int)((rc >> 16) & 0xFFu), ny = (int)(rc >> 24) - ya + 1;
+                    const int dy = 64 / nx, dx = 64 - dy * nx;
+                    int i = (int)__lane_id(), yy = i / nx, xx = i - yy * nx;
+                    for (; i < nx * ny; i += 64) {
+                        const int x = xa + xx, y = ya + yy;
+                        const float px = (float)(x + (g.tc0 >> 1)), py = (float)(y + g.tr0);
+                        const float w0 = A0 * (py - y2) - B0 * (px - x2);
+                        const float w1 = A1 * (py - y3) - B1 * (px - x3);
+                        const float w2 = A2 * (py - y1) - B2 * (px - x1);
+                        const float z = z1 + a * (w1 * dz2 + w2 * dz3);
+                        if (inside(w0, w1, w2) && z < F32_MAX)
+                            atomicMin((unsigned long long*)&keys[(y << sws) + x], (unsigned long long)make_key(z, T, 1u));
+                        xx += dx;
+                        yy += dy;
+                        if (xx >= nx) {
+                            xx -= nx;
+                            ++yy;
+                        }
+                    }
+                }
             }
             __syncthreads();
             if (multi)
@@ -663,6 +757,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
 
 // One workgroup per tile; grid (tile x, tile y, frame - f_lo).  Blocks past a
 // smaller frame's tile grid return at once.
+template <bool BIG>
 __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
     const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
     int tws, int ths, const uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
@@ -684,7 +779,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_tile_raster(
     if (n > 0)
         for (uint32_t i = threadIdx.x; i < min(n_ramps, LDS_RAMPS) * (uint32_t)sizeof(Ramp) / 4; i += blockDim.x)
             ((uint32_t*)s_ramp)[i] = ((const uint32_t*)ramps)[i];
-    raster_tile(draws, ramps, s_ramp, fr, tx, ty, o, n, tws, ths, pool, nl, careful_box, cells, zbuf, win, win_raw,
+    raster_tile<BIG>(draws, ramps, s_ramp, fr, tx, ty, o, n, tws, ths, pool, nl, careful_box, cells, zbuf, win, win_raw,
                 smem);
 }
 
@@ -734,7 +829,7 @@ int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
     if ((rc = d_off.reserve(sizeof(uint64_t) * (n_tiles + 1)))) return rc;
     if ((rc = d_nl.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_rows)))) return rc;
     if ((rc = d_overflow.reserve(sizeof(uint32_t) * std::max<size_t>(1, frames.size())))) return rc;
-    if ((rc = d_careful_cnt.reserve(sizeof(uint32_t) * (2 + MAX_GROUPS)))) return rc;
+    if ((rc = d_careful_cnt.reserve(sizeof(uint32_t) * (BIG_CNT + 1)))) return rc;
     return SLOTH_OK;
 }
 
@@ -809,9 +904,16 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     // 1. count tile entries of CLS_BOX triangles and the careful triangles
     if ((rc = launch_clear(frames, 0, F, s))) return rc;
     if ((rc = launch_bin(frames, draws, d_frames, d_draws, 0, F, true, s))) return rc;
-    uint32_t n_careful = 0;
-    HIP_TRY(hipMemcpyAsync(&n_careful, d_careful_cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    uint32_t cnt[BIG_CNT + 1];
+    HIP_TRY(hipMemcpyAsync(cnt, d_careful_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    const uint32_t n_careful = cnt[0];
+    // the wavefront walk of big boxes pays once about one wavefront in ten
+    // holds one (a big box costs its lane's wavefront 10x the common walk)
+    uint64_t n_tri = 0;
+    for (const Draw& d : draws) n_tri += d.n;
+    big_walk = (uint64_t)cnt[BIG_CNT] * 512 >= n_tri && cnt[BIG_CNT] > 0;
+    if (const char* e = getenv("SLOTH_BIG_WALK")) big_walk = e[0] == '1';
     if (n_careful > careful_cap) {
         // 2. room for the careful list, then count again so that it is filled
         careful_cap = n_careful + n_careful / 4 + 256;
@@ -854,10 +956,10 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
             my = std::max(my, frames[f].tiles_y);
         }
         if (mx == 0 || my == 0) continue;
-        hipLaunchKernelGGL(k_tile_raster, dim3(mx, my, f1 - f0), dim3(RASTER_BLOCK), raster_lds(), s, d_draws, d_ramps,
-                           d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(), d_off.as<uint64_t>(),
-                           d_pool.as<uint32_t>(), d_nl.as<int32_t>(), d_careful_box.as<uint4>(), cells, z, win,
-                           win_raw ? 1u : 0u, n_ramps);
+        hipLaunchKernelGGL(big_walk ? k_tile_raster<true> : k_tile_raster<false>, dim3(mx, my, f1 - f0),
+                           dim3(RASTER_BLOCK), raster_lds(), s, d_draws, d_ramps, d_frames, f0, log2u(TW), log2u(TH),
+                           d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
+                           d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps);
         HIP_TRY(hipGetLastError());
     }
     return SLOTH_OK;

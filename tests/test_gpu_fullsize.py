@@ -6,7 +6,8 @@
   f32::MAX, newline cells only in column 1, glyphs from the shader's
   alphabet); and a second replay byte-identical to the first;
 * the 4K frame (1,310,720 triangles, 3840x2160) against the oracle;
-* the uneven 64-mesh batch: every fourth mesh against the oracle.
+* the uneven 64-mesh batch: every fourth mesh against the oracle, with the
+  raster variant the plan picks and with the other one.
 """
 from __future__ import annotations
 
@@ -82,8 +83,16 @@ def test_frame_4k_full(sloth):
     _frame_properties(cells, z, W, H)
 
 
-def test_uneven_batch_full(sloth):
+@pytest.mark.parametrize("big_walk", ["auto", "0"])
+def test_uneven_batch_full(sloth, big_walk, monkeypatch):
+    """The plan picks the big-box raster variant for this batch (about 3% of
+    its triangles have boxes of more than 128 pixels); "0" forces the other."""
     from sloth_amd import meshgen
+
+    if big_walk == "auto":
+        monkeypatch.delenv("SLOTH_BIG_WALK", raising=False)
+    else:
+        monkeypatch.setenv("SLOTH_BIG_WALK", big_walk)
 
     items = meshgen.workload_uneven_meshes()
     frames, draws, keep = [], [], []
@@ -92,7 +101,7 @@ def test_uneven_batch_full(sloth):
         keep.append(sm)
         frames.append(sloth.FrameSpec(W, H, U, True))
         draws.append(sloth.DrawSpec(i, sm, R))
-    say(f"case uneven batch: 64 meshes, {sum(it[0].n for it in items)} triangles")
+    say(f"case uneven batch (big walk {big_walk}): 64 meshes, {sum(it[0].n for it in items)} triangles")
     b = sloth.Batch(frames, draws)
     b.run()
     b.sync()

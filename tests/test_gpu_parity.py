@@ -16,14 +16,17 @@ from _helpers import assert_frame_equal, gpu_render, oracle_render, random_soup,
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["tiled", "global"])
+@pytest.fixture(autouse=True, params=["tiled", "tiled-big", "global"])
 def raster_path(request, monkeypatch):
-    """Every case runs on the tiled path (even widths) and on the general
+    """Every case runs on the tiled path (even widths) with each tile raster
+    variant (SLOTH_BIG_WALK=0: every lane walks its own box; 1: wavefronts
+    walk boxes of more than 128 pixels together) and on the general
     global-key path (forced by SLOTH_FORCE_GLOBAL=1)."""
     if request.param == "global":
         monkeypatch.setenv("SLOTH_FORCE_GLOBAL", "1")
     else:
         monkeypatch.delenv("SLOTH_FORCE_GLOBAL", raising=False)
+    monkeypatch.setenv("SLOTH_BIG_WALK", "1" if request.param == "tiled-big" else "0")
     return request.param
 
 
@@ -31,6 +34,8 @@ def _check(sloth, name, W, H, U, draws, image=True):
     import os
 
     path = "global" if os.environ.get("SLOTH_FORCE_GLOBAL") == "1" or W % 2 else "tiled"
+    if path == "tiled" and os.environ.get("SLOTH_BIG_WALK") == "1":
+        path = "tiled-big"
     say(f"case {name} [{path}]: {W}x{H} image={image} draws={len(draws)} "
         f"tris={sum(d[0].shape[0] for d in draws)}")
     cells, z, win = gpu_render(sloth, W, H, U, draws, image=image)
