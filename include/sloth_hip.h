@@ -150,12 +150,14 @@ int sloth_draw_mesh(sloth_context *ctx, const sloth_mesh *mesh, const float *tra
  * stream).  Returns without waiting for the device. */
 int sloth_context_render(sloth_context *ctx, void *stream);
 
-/* Renders if needed, then copies the frame buffer, z buffer and winning
- * triangle ids to the host (any pointer may be NULL) and waits. */
+/* Renders if needed, then copies the frame buffer (W*H (+H) cells), z buffer
+ * and winning triangle ids (W*H each) to the host (any pointer may be NULL)
+ * and waits. */
 int sloth_context_read(sloth_context *ctx, uint32_t *cells, float *z, int32_t *winners);
 
 /* Device addresses of the context's cell and z buffers (valid until the
- * context is destroyed), and their element counts. */
+ * context is destroyed), and the frame's element counts: W*H (+H) cells and
+ * W*H depths. */
 int sloth_context_device_buffers(sloth_context *ctx, uint32_t **d_cells, float **d_z, uint64_t *n_cells,
                                  uint64_t *n_z);
 
@@ -221,7 +223,8 @@ int sloth_batch_read_frame(sloth_batch *b, uint32_t frame, uint32_t *cells, floa
 /* sloth_context_encode for one frame of a batch that has been run. */
 int sloth_batch_encode_frame(sloth_batch *b, uint32_t frame, int mode, char *out, uint64_t cap, uint64_t *len);
 
-/* Device addresses of one frame's cells and z. */
+/* Device addresses of one frame's cells (W*H (+H) of them, starting on a
+ * 256-byte boundary) and z (W*H). */
 int sloth_batch_device_frame(sloth_batch *b, uint32_t frame, uint32_t **d_cells, float **d_z);
 
 /* Times each stage of one (non-graph) run with HIP events on the batch's

@@ -247,7 +247,11 @@ struct Job {
             if (tri >= (1u << 30)) small_frames = false;
             uint64_t WH = (uint64_t)fr.W * fr.H;
             cells += WH + (fr.image ? fr.H : 0);
-            cells = (cells + 1) & ~1ull;   // even offsets: slots store cell pairs as one 8-byte word
+            // frames start on 256-byte boundaries: a frame's rows of tile
+            // stores then cover whole cache lines (an odd frame base costs
+            // the streaming stores a third more lines per row segment), and
+            // slots store cell pairs as one 8-byte word
+            cells = (cells + 63) & ~63ull;
             z += WH;
             rows += fr.H;
             if (fr.W & 1u) all_even = false;
@@ -324,7 +328,7 @@ struct Job {
         uint64_t it_lo = d_lo < draws.size() ? draws[d_lo].item_begin : n_items;
         uint64_t it_hi = d_hi < draws.size() ? draws[d_hi].item_begin : n_items;
         uint64_t c_lo = a.cell_begin;
-        uint64_t c_hi = b.cell_begin + (uint64_t)b.W * b.H + (b.image ? b.H : 0);
+        uint64_t c_hi = b.cell_begin + frame_cells(b);
         if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
         {
             uint64_t n = std::max(z_hi - z_lo, r_hi - r_lo);
@@ -836,8 +840,8 @@ int sloth_context_read(sloth_context* ctx, uint32_t* cells, float* z, int32_t* w
         ctx->checked = true;
     }
     const Job& j = ctx->job;
-    if (cells && j.n_cells)
-        HIP_TRY(hipMemcpyAsync(cells, j.d_cells.p, sizeof(uint32_t) * j.n_cells, hipMemcpyDeviceToHost, s));
+    const uint64_t nc = frame_cells(j.frames[0]);   // the buffer is padded past the frame's cells
+    if (cells && nc) HIP_TRY(hipMemcpyAsync(cells, j.d_cells.p, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost, s));
     if (z && j.n_z) HIP_TRY(hipMemcpyAsync(z, j.d_z.p, sizeof(float) * j.n_z, hipMemcpyDeviceToHost, s));
     if (winners && j.n_z)
         HIP_TRY(hipMemcpyAsync(winners, j.d_win.p, sizeof(int32_t) * j.n_z, hipMemcpyDeviceToHost, s));
@@ -850,10 +854,10 @@ int sloth_context_encode(sloth_context* ctx, int mode, char* out, uint64_t cap, 
     ARG_CHECK(ctx, "null context");
     int rc = sloth_context_read(ctx, nullptr, nullptr, nullptr);   // render + overflow check
     if (rc) return rc;
-    // the frame's own cells (the layout pads the buffer to an even count)
+    // the frame's own cells (the layout pads the buffer past them)
     const Frame& fr = ctx->job.frames[0];
-    const uint64_t nc = (uint64_t)fr.W * fr.H + (fr.image ? fr.H : 0);
-    return ctx->enc.run(ctx->job.d_cells.as<uint32_t>(), nc, fr.image != 0, mode, out, cap, len, ctx->stream);
+    return ctx->enc.run(ctx->job.d_cells.as<uint32_t>(), frame_cells(fr), fr.image != 0, mode, out, cap, len,
+                        ctx->stream);
 }
 
 int sloth_context_device_buffers(sloth_context* ctx, uint32_t** d_cells, float** d_z, uint64_t* n_cells,
@@ -866,7 +870,7 @@ int sloth_context_device_buffers(sloth_context* ctx, uint32_t** d_cells, float**
     }
     if (d_cells) *d_cells = ctx->job.d_cells.as<uint32_t>();
     if (d_z) *d_z = ctx->job.d_z.as<float>();
-    if (n_cells) *n_cells = ctx->job.n_cells;
+    if (n_cells) *n_cells = frame_cells(ctx->job.frames[0]);
     if (n_z) *n_z = ctx->job.n_z;
     return SLOTH_OK;
 }
@@ -1032,8 +1036,7 @@ int sloth_batch_read_frame(sloth_batch* b, uint32_t frame, uint32_t* cells, floa
     if (!b->ran) return fail(SLOTH_ESTATE, "batch has not been run");
     if ((rc = sloth_batch_sync(b))) return rc;
     const Frame& fr = b->job.frames[frame];
-    uint64_t WH = (uint64_t)fr.W * fr.H;
-    uint64_t nc = WH + (fr.image ? fr.H : 0);
+    const uint64_t WH = (uint64_t)fr.W * fr.H, nc = frame_cells(fr);
     if (winners && !b->win_valid[frame]) {
         if ((rc = b->job.ensure_win())) return rc;
         if ((rc = b->job.launch(frame, frame + 1, s, b->job.d_win.as<int32_t>()))) return rc;
@@ -1056,7 +1059,7 @@ int sloth_batch_encode_frame(sloth_batch* b, uint32_t frame, int mode, char* out
     int rc = sloth_batch_sync(b);
     if (rc) return rc;
     const Frame& fr = b->job.frames[frame];
-    const uint64_t nc = (uint64_t)fr.W * fr.H + (fr.image ? fr.H : 0);
+    const uint64_t nc = frame_cells(fr);
     return b->enc.run(b->job.d_cells.as<uint32_t>() + fr.cell_off, nc, fr.image != 0, mode, out, cap, len, b->stream);
 }
 

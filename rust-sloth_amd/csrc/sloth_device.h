@@ -43,13 +43,20 @@ struct Frame {
     uint32_t image;
     uint32_t draw_begin, draw_end;   // [begin, end) into the draw table
     uint32_t n_tris;
-    uint64_t cell_off;    // into the cell buffer, W*H (+H) cells
+    uint64_t cell_off;    // into the cell buffer (a multiple of 64), W*H (+H) cells
     uint64_t z_off;       // into the z / key / winner buffers, W*H slots
     uint64_t nl_off;      // into the per-row newline table, H rows
     uint64_t cell_begin;  // index of this frame's first cell among all frames
     uint32_t tiles_x, tiles_y;   // tiled path: tile grid over the W x H cells
     uint64_t tile_begin;         // tiled path: index of the frame's first tile
 };
+
+// Cells of a frame: W*H, plus one newline cell per row in image mode
+// (context.rs:38-40).  Frames are laid out further apart than this.
+__host__ __device__ inline uint64_t frame_cells(const Frame& f)
+{
+    return (uint64_t)f.W * f.H + (f.image ? f.H : 0);
+}
 
 // Transformed triangle + the quantities draw_triangle derives from it.
 struct Tri {

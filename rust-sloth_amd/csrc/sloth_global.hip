@@ -141,6 +141,7 @@ __global__ void __launch_bounds__(256) k_resolve_global(const Draw* __restrict__
     const uint32_t W = fr.W, H = fr.H;
     const uint64_t WH = (uint64_t)W * H;
     const uint64_t c = g - fr.cell_begin;
+    if (c >= frame_cells(fr)) return;   // padding between frames
     const uint64_t* fk = keys + fr.z_off;
 
     uint64_t best_time = 0;   // 0 = no event; event times are stored +1
