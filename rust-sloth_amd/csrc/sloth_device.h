@@ -126,9 +126,13 @@ __device__ __forceinline__ void edges(const Tri& t, float px, float py, float& w
     w2 = t.A2 * (py - t.y1) - t.B2 * (px - t.x1);
 }
 
+// w0 >= 0 && w1 >= 0 && w2 >= 0 (rasterizer.rs:80), as one v_minimum3_f32 and
+// one compare: IEEE 754-2019 minimum returns NaN when any operand is NaN (then
+// the compare fails, as `NaN >= 0` does) and -0 for (-0, +0) (which passes, as
+// `-0 >= 0` does), so the two forms agree on every input.
 __device__ __forceinline__ bool inside(float w0, float w1, float w2)
 {
-    return w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f;
+    return __builtin_elementwise_minimum(__builtin_elementwise_minimum(w0, w1), w2) >= 0.0f;
 }
 
 // Depth (rasterizer.rs:77-79).
@@ -180,9 +184,10 @@ __device__ __forceinline__ uint8_t ramp_glyph(const Ramp& r, float s)
 // -0 and +0 compare equal in the reference's `<`, so both map to +0's code.
 __device__ __forceinline__ uint32_t depth_code(float z)
 {
-    uint32_t u = __float_as_uint(z);
-    if (z == 0.0f) u = 0u;
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    // z + 0 turns -0 into +0 (and changes no other value), so that -0 == +0;
+    // then negative values have every bit flipped and the others the sign bit
+    const uint32_t u = __float_as_uint(z + 0.0f);
+    return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
 }
 
 // Packed z-test key: (depth code, frame triangle index, sub).  `sub` orders the

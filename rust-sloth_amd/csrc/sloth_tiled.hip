@@ -449,7 +449,22 @@ struct TileGeom {
     int W;
 };
 
-// Calls f(x, y, slot, sub) for every logical pixel of the box that lands in
+// Calls f(px, py, slot, sub) for the pixels (x, y) in [xa, xb] x [ya, yb],
+// px = x and py = y as floats, slot = row0 + ((y - ya) << sws) + (x - xa).
+// Pixel coordinates are below 2^16, so stepping px by 1.0f is exact (one add
+// instead of a conversion per pixel).
+template <class Fn>
+__device__ __forceinline__ void walk_rect(int xa, int xb, int ya, int yb, int row0, int sws, uint32_t sub, Fn&& f)
+{
+    const float fxb = (float)xb;
+    for (int y = ya; y <= yb; ++y, row0 += 1 << sws) {
+        const float py = (float)y;
+        int slot = row0;
+        for (float px = (float)xa; px <= fxb; px += 1.0f, ++slot) f(px, py, slot, sub);
+    }
+}
+
+// Calls f(px, py, slot, sub) for every logical pixel of the box that lands in
 // the tile's wrapped region (2x - W in [tc0,tc1), row y + 1).
 template <class Fn>
 __device__ __forceinline__ void walk_wrapped(const TileGeom& g, int bx0, int bx1, int by0, int by1, Fn&& f)
@@ -457,19 +472,17 @@ __device__ __forceinline__ void walk_wrapped(const TileGeom& g, int bx0, int bx1
     const int wx0 = (g.tc0 + g.W) >> 1;
     const int xa = max(bx0, wx0), xb = min(bx1, ((g.tc1 + g.W) >> 1) - 1);
     const int ya = max(by0, g.tr0 - 1), yb = min(by1, g.tr1 - 2);
-    for (int y = ya; y <= yb; ++y)
-        for (int x = xa; x <= xb; ++x) f(x, y, ((y + 1 - g.tr0) << g.sws) + (x - wx0), 0u);
+    walk_rect(xa, xb, ya, yb, ((ya + 1 - g.tr0) << g.sws) + (xa - wx0), g.sws, 0u, f);
 }
 
-// Calls f(x, y, slot, sub) for every logical pixel of the box that lands in
+// Calls f(px, py, slot, sub) for every logical pixel of the box that lands in
 // the tile: the direct region (2x in [tc0,tc1), row y), then the wrapped one.
 template <class Fn>
 __device__ __forceinline__ void walk_tile(const TileGeom& g, int bx0, int bx1, int by0, int by1, Fn&& f)
 {
     const int xa = max(bx0, g.tc0 >> 1), xb = min(bx1, (g.tc1 >> 1) - 1);
     const int ya = max(by0, g.tr0), yb = min(by1, g.tr1 - 1);
-    for (int y = ya; y <= yb; ++y)
-        for (int x = xa; x <= xb; ++x) f(x, y, ((y - g.tr0) << g.sws) + (x - (g.tc0 >> 1)), 1u);
+    walk_rect(xa, xb, ya, yb, ((ya - g.tr0) << g.sws) + (xa - (g.tc0 >> 1)), g.sws, 1u, f);
     walk_wrapped(g, bx0, bx1, by0, by1, f);
 }
 
@@ -665,9 +678,9 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                     by1 = t.y1d - 1;
                 }
                 if (bx0 <= bx1 && by0 <= by1) {
-                    auto test = [&](int x, int y, int slot, uint32_t sub) {
+                    auto test = [&](float px, float py, int slot, uint32_t sub) {
                         float w0, w1, w2;
-                        edges(t, (float)x, (float)y, w0, w1, w2);
+                        edges(t, px, py, w0, w1, w2);
                         const float z = depth(t, w1, w2);
                         if (inside(w0, w1, w2) && z < F32_MAX)
                             atomicMin((unsigned long long*)&keys[slot], (unsigned long long)make_key(z, T, sub));
@@ -681,9 +694,8 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                             big_rect =
                                 (uint32_t)xa | ((uint32_t)xb << 8) | ((uint32_t)ya << 16) | ((uint32_t)yb << 24);
                         } else {
-                            for (int y = ya; y <= yb; ++y)
-                                for (int x = xa; x <= xb; ++x)
-                                    test(x + (g.tc0 >> 1), y + g.tr0, (y << sws) + x, 1u);
+                            walk_rect(xa + (g.tc0 >> 1), xb + (g.tc0 >> 1), ya + g.tr0, yb + g.tr0, (ya << sws) + xa,
+                                      sws, 1u, test);
                         }
                         walk_wrapped(g, bx0, bx1, by0, by1, test);
                     } else {
