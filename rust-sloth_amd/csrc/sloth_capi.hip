@@ -90,6 +90,9 @@ static void finish_ramp(Ramp& r)
         if (i + 1 < r.n && r.thr[i] > r.thr[i + 1]) ok = false;
     }
     r.sorted = ok ? 1 : 0;
+    // unused steps: +inf, which no shade exceeds (the device's binary search
+    // over all MAX_RAMP thresholds relies on it)
+    for (uint32_t i = r.n; i < MAX_RAMP; ++i) r.thr[i] = INFINITY;
 }
 
 static Ramp default_ramp()

@@ -21,7 +21,8 @@ struct Ramp {
     float thr[MAX_RAMP];
     uint8_t glyph[MAX_RAMP];
     uint8_t fallback;
-    uint8_t sorted;   // thresholds non-decreasing and none NaN (set by the host)
+    uint8_t sorted;   // thresholds non-decreasing and none NaN (set by the host,
+                      // which also sets thr[n..MAX_RAMP) to +inf)
     uint8_t pad[2];
 };
 
@@ -89,8 +90,19 @@ __device__ __forceinline__ int sat_index(float f)
 }
 
 // Triangle::clone + mul + aabb + pixel domain + a (rasterizer.rs:56-67).
-__device__ __forceinline__ void setup_tri(const float* M, const float* p, uint32_t W, uint32_t H, Tri& t)
+// Mesh arrays are device-accessible global memory, but their addresses come
+// from tables in memory, so the compiler cannot tell them from LDS and emits
+// flat loads (which wait on both memory counters).  Reading through an
+// explicit global pointer gives global loads.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* global_ptr(const T* p)
 {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+
+__device__ __forceinline__ void setup_tri(const float* M, const float* pv, uint32_t W, uint32_t H, Tri& t)
+{
+    const auto* p = global_ptr(pv);
     t.x1 = xform_row(M, 0, p[0], p[1], p[2]);
     t.y1 = xform_row(M, 1, p[0], p[1], p[2]);
     t.z1 = xform_row(M, 2, p[0], p[1], p[2]);
@@ -170,10 +182,18 @@ __device__ __forceinline__ uint8_t ramp_glyph(const Ramp& r, float s)
     if (r.sorted) {
         // Non-decreasing thresholds: {k : thr[k] < s} is a prefix [0, i), so
         // i is the first index with s <= thr[i] (none when i == n); a NaN
-        // shade passes no threshold.  Counting avoids a divergent early exit.
+        // shade passes no threshold.  The host pads thr[n..16) with +inf,
+        // which no shade exceeds, so i is found by a fixed binary search over
+        // all 16 entries (steps 8, 4, 2, 1 reach 15; the last step takes
+        // i = 15 to 16) instead of a divergent scan.
+        static_assert(MAX_RAMP == 16, "the search below covers 16 thresholds");
         uint32_t i = 0;
-        for (uint32_t k = 0; k < r.n; ++k) i += (r.thr[k] < s) ? 1u : 0u;
-        return (i == r.n || s != s) ? r.fallback : r.glyph[i];
+        i += (r.thr[i + 7] < s) ? 8u : 0u;
+        i += (r.thr[i + 3] < s) ? 4u : 0u;
+        i += (r.thr[i + 1] < s) ? 2u : 0u;
+        i += (r.thr[i] < s) ? 1u : 0u;
+        i += (r.thr[i] < s) ? 1u : 0u;
+        return (i >= r.n || s != s) ? r.fallback : r.glyph[i];
     }
     for (uint32_t i = 0; i < r.n; ++i)
         if (s <= r.thr[i]) return r.glyph[i];

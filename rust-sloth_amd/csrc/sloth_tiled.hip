@@ -605,7 +605,13 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         // newline cells of column 1
         for_my_slots([&](uint32_t, int row, uint64_t id) { store(row, id, KEY_EMPTY, CELL_BLANK, F32_MAX); });
     } else {
-        auto ramp_of = [&](uint32_t r) -> const Ramp& { return r < LDS_RAMPS ? s_ramp[r] : ramps[r]; };
+        // glyph of shade sh under shader r: the first LDS_RAMPS shaders from
+        // LDS, the others from global memory (two calls, so that each reads
+        // its own address space rather than through generic pointers)
+        auto glyph_of = [&](uint32_t r, float sh) -> uint32_t {
+            if (r < LDS_RAMPS) return ramp_glyph(s_ramp[r], sh);
+            return ramp_glyph(ramps[r], sh);
+        };
         uint64_t* keys = smem;
         uint32_t* htag = (uint32_t*)(keys + S);
         float* cache = (float*)(htag + HASH_CAP);
@@ -637,7 +643,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             z = q1.z + q2.y * (w1 * q1.w + w2 * q2.x);
             const float sh = q2.z * (w0 + w1 + w2);
             const uint32_t cw = __float_as_uint(q2.w);
-            cell = (uint32_t)ramp_glyph(ramp_of(cw & 0xFFu), sh) | (cw & 0xFFFFFF00u);
+            cell = glyph_of(cw & 0xFFu, sh) | (cw & 0xFFFFFF00u);
             return true;
         };
 
@@ -658,7 +664,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 Tri t;
                 setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
                 const float na = unit_normal_z(t) * t.a;   // rasterizer.rs:76 multiplies these first
-                const uint8_t* c = dr.rgb + (size_t)li * 3;
+                const auto* c = global_ptr(dr.rgb + (size_t)li * 3);
                 const uint32_t cw = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24) | dr.ramp;
                 float4* row4 = (float4*)(cache + p * CACHE_WORDS);
                 row4[0] = make_float4(t.x1, t.y1, t.x2, t.y2);

@@ -85,8 +85,14 @@ def test_custom_shaders_per_draw(sloth):
     sh3 = Shader((0.9, 0.3, 0.6, 0.3, 0.95), "vwxyz", "q")
     # non-decreasing with repeats and an infinite step
     sh4 = Shader((0.25, 0.25, 0.5, float("inf")), "abcd", "e")
+    # the full 16 steps (shades above the last one take the fallback), no
+    # step at all, and one step; with 8 shaders in the job, the ones past the
+    # first four are read from global memory rather than LDS
+    sh5 = Shader(tuple(-0.05 + 0.0625 * k for k in range(16)), "ABCDEFGHIJKLMNOP", "!")
+    sh6 = Shader((), "", "0")
+    sh7 = Shader((0.5,), "1", "2")
     draws = []
-    for sh in (sh1, sh2, sh3, sh4, None):
+    for sh in (sh1, sh2, sh3, sh4, sh5, sh6, sh7, None):
         xyz, rgb = random_soup(rng, 300, W, H, U, size=0.1)
         draws.append((xyz, rgb, meshgen.rotation_euler(0.2, 1.1, 0.3), sh))
     _check(sloth, "per-draw shaders", W, H, U, draws)
