@@ -218,11 +218,6 @@ __device__ __forceinline__ uint64_t make_key(float z, uint32_t tri, uint32_t sub
     return ((uint64_t)depth_code(z) << 32) | (uint64_t)((tri << 1) | sub);
 }
 
-__device__ __forceinline__ uint32_t pack_cell(uint8_t g, const uint8_t* rgb)
-{
-    return (uint32_t)g | ((uint32_t)rgb[0] << 8) | ((uint32_t)rgb[1] << 16) | ((uint32_t)rgb[2] << 24);
-}
-
 // Logical pixel (x, y) that a key with `sub` denotes at z slot `id`.
 __device__ __forceinline__ void key_pixel(uint32_t id, uint32_t sub, uint32_t W, uint32_t& x, uint32_t& y)
 {

@@ -118,7 +118,9 @@ __device__ __forceinline__ void winner_fragment(const Draw* __restrict__ draws, 
     edges(t, (float)px, (float)py, w0, w1, w2);
     z = depth(t, w1, w2);
     float s = shade_value(unit_normal_z(t), t.a, w0, w1, w2);
-    cell = pack_cell(ramp_glyph(ramps[dr.ramp], s), dr.rgb + (size_t)local * 3);
+    const auto* c = global_ptr(dr.rgb + (size_t)local * 3);
+    cell = (uint32_t)ramp_glyph(ramps[dr.ramp], s) | ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) |
+           ((uint32_t)c[2] << 24);
 }
 
 __global__ void __launch_bounds__(256) k_resolve_global(const Draw* __restrict__ draws, const Ramp* __restrict__ ramps,
