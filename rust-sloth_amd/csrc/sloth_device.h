@@ -282,6 +282,61 @@ __device__ __forceinline__ uint32_t setup_class(const Tri& t)
     return CLS_SPANS;
 }
 
+// Per-row candidate spans of a CLS_BOX / CLS_TIGHT triangle, for walking a
+// box [bx0, bx1] x [by0, by1] row by row from row y_first over at most 64 rows.
+//
+// Edge k (anchor (xk, yk), coefficients A, B as in Tri) passes in FP32 only
+// where its exact value E = A (py - yk) - B (px - xk) is >= -err, with err <=
+// 3.0001 u (|A| |py - yk| + |B| |px - xk|) (three roundings of the
+// reference's expression, u = 2^-24) <= e = 4u (|A| Dy + |B| Dx) over the box
+// (Dx, Dy the largest distances from the anchor).  So a passing pixel has
+// px <= T(py) = xk + (A (py - yk) + e) / B when B > 0, and px >= T(py) when
+// B < 0; B = 0 gives no limit.  T is linear in py (slope s = A / B): it is
+// evaluated at y_first and stepped by s per row.  The float evaluation (a
+// hardware reciprocal within 1 ulp, a few roundings, up to 64 steps) errs by
+// well under 2^-17 of the magnitudes involved (M below); the limits are moved
+// outward by d = 2^-13 M + 2^-12 pixel, so every pixel that can pass lies
+// within [ceil(max L), floor(min H)] of its row.  Overflow or NaN in any of
+// these values only removes the limit (min / max ignore NaN; +-inf limits are
+// no limit), so the span is never narrower than the passing set; the walk
+// still evaluates the reference's test on every pixel of it.
+struct RowSpans {
+    float H[3], L[3], s[3];   // per edge: upper / lower limit at the current row (or +-inf), step per row
+};
+
+__device__ __forceinline__ void row_spans_begin(const Tri& t, int bx0, int bx1, int by0, int by1, int y_first,
+                                                RowSpans& r)
+{
+    const float ax[3] = {t.x2, t.x3, t.x1}, ay[3] = {t.y2, t.y3, t.y1};
+    const float A[3] = {t.A0, t.A1, t.A2}, B[3] = {t.B0, t.B1, t.B2};
+    const float X0 = (float)bx0, X1 = (float)bx1, Y0 = (float)by0, Y1 = (float)by1, Yf = (float)y_first;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float rB = __builtin_amdgcn_rcpf(B[k]);
+        const float sk = A[k] * rB;
+        const float Dy = fmaxf(fabsf(Y0 - ay[k]), fabsf(Y1 - ay[k]));
+        const float Dx = fmaxf(fabsf(X0 - ax[k]), fabsf(X1 - ax[k]));
+        const float e = 0x1p-22f * (fabsf(A[k]) * Dy + fabsf(B[k]) * Dx) + 0x1p-120f;
+        const float T = ax[k] + (A[k] * (Yf - ay[k]) + e) * rB;
+        const float M = fabsf(ax[k]) + (fabsf(A[k]) * Dy + e) * fabsf(rB) + fabsf(sk) * 64.0f;
+        const float d = 0x1p-13f * M + 0x1p-12f;
+        r.s[k] = sk;
+        r.H[k] = B[k] > 0.0f ? T + d : INFINITY;
+        r.L[k] = B[k] < 0.0f ? T - d : -INFINITY;
+    }
+}
+
+// No limits (careful entries, whose boxes k_careful computed).
+__device__ __forceinline__ void row_spans_none(RowSpans& r)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.s[k] = 0.0f;
+        r.H[k] = INFINITY;
+        r.L[k] = -INFINITY;
+    }
+}
+
 // x range of the candidate box of a CLS_BOX / CLS_TIGHT triangle, clipped to
 // its domain (empty when bx0 > bx1).  |minx|, |maxx| < 2^22 for both classes.
 __device__ __forceinline__ void box_x(const Tri& t, bool tight, int& bx0, int& bx1)

@@ -429,6 +429,7 @@ constexpr uint32_t CHUNK = RASTER_BLOCK;   // list entries whose triangles sit i
 constexpr uint32_t HASH_CAP = 512;         // open-addressing table: triangle -> cache row
 constexpr uint32_t CACHE_WORDS = 12;
 constexpr uint32_t LDS_RAMPS = 4;          // shaders staged in LDS per tile
+constexpr uint32_t SPAN_MIN_W = 6;         // box width (pixels) from which a wavefront walks row spans
 static_assert(CHUNK <= 256, "hash rows hold a cache row index in one byte");
 
 __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws, const Frame& fr, uint32_t tri)
@@ -464,6 +465,30 @@ __device__ __forceinline__ void walk_rect(int xa, int xb, int ya, int yb, int ro
     }
 }
 
+// walk_rect over the direct region with each row cut to its candidate span
+// (row_spans_begin, started at row ya): slot = row0 + ((y - ya) << sws) +
+// (x - xa) as there.
+template <class Fn>
+__device__ __forceinline__ void walk_rect_spans(int xa, int xb, int ya, int yb, int row0, int sws, RowSpans& r,
+                                                Fn&& f)
+{
+    const float fxa = (float)xa, fxb = (float)xb, fxe = fxb + 1.0f;
+    for (int y = ya; y <= yb; ++y, row0 += 1 << sws) {
+        // lo within [xa, xb + 1] (an empty row when above hi), so that its
+        // integer conversion is exact
+        const float lo = fminf(fmaxf(fmaxf(fmaxf(r.L[0], r.L[1]), r.L[2]), fxa), fxe);
+        const float hi = fminf(fminf(fminf(r.H[0], r.H[1]), r.H[2]), fxb);
+        const float py = (float)y, px0 = ceilf(lo), px1 = floorf(hi);
+        int slot = row0 + ((int)px0 - xa);
+        for (float px = px0; px <= px1; px += 1.0f, ++slot) f(px, py, slot, 1u);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            r.H[k] += r.s[k];
+            r.L[k] += r.s[k];
+        }
+    }
+}
+
 // Calls f(px, py, slot, sub) for every logical pixel of the box that lands in
 // the tile's wrapped region (2x - W in [tc0,tc1), row y + 1).
 template <class Fn>
@@ -475,14 +500,19 @@ __device__ __forceinline__ void walk_wrapped(const TileGeom& g, int bx0, int bx1
     walk_rect(xa, xb, ya, yb, ((ya + 1 - g.tr0) << g.sws) + (xa - wx0), g.sws, 0u, f);
 }
 
-// Calls f(px, py, slot, sub) for every logical pixel of the box that lands in
-// the tile: the direct region (2x in [tc0,tc1), row y), then the wrapped one.
+// Calls f(px, py, slot, sub) for the candidate pixels of the box that land in
+// the tile: the direct region (2x in [tc0,tc1), row y) with each row cut to
+// its span (spans started at row max(by0, tr0)), then the wrapped one.
 template <class Fn>
-__device__ __forceinline__ void walk_tile(const TileGeom& g, int bx0, int bx1, int by0, int by1, Fn&& f)
+__device__ __forceinline__ void walk_tile(const TileGeom& g, int bx0, int bx1, int by0, int by1, bool use_spans,
+                                          RowSpans& r, Fn&& f)
 {
     const int xa = max(bx0, g.tc0 >> 1), xb = min(bx1, (g.tc1 >> 1) - 1);
     const int ya = max(by0, g.tr0), yb = min(by1, g.tr1 - 1);
-    walk_rect(xa, xb, ya, yb, ((ya - g.tr0) << g.sws) + (xa - (g.tc0 >> 1)), g.sws, 1u, f);
+    if (use_spans)
+        walk_rect_spans(xa, xb, ya, yb, ((ya - g.tr0) << g.sws) + (xa - (g.tc0 >> 1)), g.sws, r, f);
+    else
+        walk_rect(xa, xb, ya, yb, ((ya - g.tr0) << g.sws) + (xa - (g.tc0 >> 1)), g.sws, 1u, f);
     walk_wrapped(g, bx0, bx1, by0, by1, f);
 }
 
@@ -678,10 +708,19 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                     }
                     h = (h + 1) & (HASH_CAP - 1);
                 }
+                RowSpans spans;
                 if (!given) {
                     box_x(t, tight, bx0, bx1);
                     by0 = t.y0;
                     by1 = t.y1d - 1;
+                }
+                // row spans cost set-up and per-row work that only boxes of
+                // some width repay; a wavefront walks as long as its widest
+                // box, so it takes them when any lane's box is that wide
+                const bool use_spans = __ballot(bx1 - bx0 + 1 >= (int)SPAN_MIN_W) != 0;
+                if (use_spans) {
+                    if (!given) row_spans_begin(t, bx0, bx1, by0, by1, max(by0, g.tr0), spans);
+                    else row_spans_none(spans);
                 }
                 if (bx0 <= bx1 && by0 <= by1) {
                     auto test = [&](float px, float py, int slot, uint32_t sub) {
@@ -700,12 +739,16 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                             big_rect =
                                 (uint32_t)xa | ((uint32_t)xb << 8) | ((uint32_t)ya << 16) | ((uint32_t)yb << 24);
                         } else {
-                            walk_rect(xa + (g.tc0 >> 1), xb + (g.tc0 >> 1), ya + g.tr0, yb + g.tr0, (ya << sws) + xa,
-                                      sws, 1u, test);
+                            if (use_spans)
+                                walk_rect_spans(xa + (g.tc0 >> 1), xb + (g.tc0 >> 1), ya + g.tr0, yb + g.tr0,
+                                                (ya << sws) + xa, sws, spans, test);
+                            else
+                                walk_rect(xa + (g.tc0 >> 1), xb + (g.tc0 >> 1), ya + g.tr0, yb + g.tr0,
+                                          (ya << sws) + xa, sws, 1u, test);
                         }
                         walk_wrapped(g, bx0, bx1, by0, by1, test);
                     } else {
-                        walk_tile(g, bx0, bx1, by0, by1, test);
+                        walk_tile(g, bx0, bx1, by0, by1, use_spans, spans, test);
                     }
                 }
             }

@@ -169,6 +169,45 @@ def test_slivers_near_silhouette(sloth):
     _check(sloth, "slivers", W, H, I, [(xyz, rgb, I, None)])
 
 
+@pytest.mark.parametrize("kind", ["flat-edges", "steep-edges", "wide", "offscreen"])
+def test_row_span_limits(sloth, kind):
+    """Triangles whose boxes are at least 6 pixels wide, so that the tiled
+    walk cuts each row to its span: edges within 1e-3 of horizontal or
+    vertical, wide slivers, large triangles reaching far off screen.  Every
+    pixel the FP32 test passes must lie inside the computed spans."""
+    rng = np.random.default_rng({"flat-edges": 11, "steep-edges": 12, "wide": 13, "offscreen": 14}[kind])
+    W, H = 240, 90
+    I = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1], np.float32)
+    n = 1500
+    c = rng.uniform([0, 0], [W / 2, H], (n, 2))
+    if kind == "flat-edges":
+        # one edge nearly horizontal, the third vertex well above or below
+        a = c + rng.uniform(-20, 20, (n, 2)) * [1, 0] + rng.normal(0, 1e-3, (n, 2)) * [0, 1]
+        b = c + rng.uniform(-20, 20, (n, 2)) * [1, 0] + rng.normal(0, 1e-3, (n, 2)) * [0, 1]
+        d = c + rng.uniform(-15, 15, (n, 2))
+    elif kind == "steep-edges":
+        a = c + rng.uniform(-20, 20, (n, 2)) * [0, 1] + rng.normal(0, 1e-3, (n, 2)) * [1, 0]
+        b = c + rng.uniform(-20, 20, (n, 2)) * [0, 1] + rng.normal(0, 1e-3, (n, 2)) * [1, 0]
+        d = c + rng.uniform(-15, 15, (n, 2))
+    elif kind == "wide":
+        # long thin triangles in every direction
+        ang = rng.uniform(0, 2 * math.pi, n)
+        L = rng.uniform(8, 60, n)
+        u = np.stack([np.cos(ang), np.sin(ang)], 1)
+        a = c - u * L[:, None] / 2
+        b = c + u * L[:, None] / 2
+        d = c + np.stack([-u[:, 1], u[:, 0]], 1) * rng.normal(0, 0.8, (n, 1))
+    else:
+        # large triangles with vertices up to 5000 pixels away
+        a = c + rng.normal(0, 1500, (n, 2))
+        b = c + rng.uniform(-30, 30, (n, 2))
+        d = c + rng.uniform(-30, 30, (n, 2))
+    z = rng.uniform(-10, 10, (n, 3, 1))
+    xyz = np.concatenate([np.stack([a, b, d], 1), z], 2).astype(np.float32)
+    rgb = rng.integers(0, 256, (n, 3), dtype=np.uint8)
+    _check(sloth, f"row spans {kind}", W, H, I, [(xyz, rgb, I, None)])
+
+
 def test_coplanar_ties(sloth):
     """Identical triangles drawn twice: the first one drawn keeps every z slot
     (strict `<`, rasterizer.rs:81)."""
