@@ -258,9 +258,11 @@ __device__ __forceinline__ uint32_t setup_class(const Tri& t)
 {
     if (t.x0 >= t.x1d || t.y0 >= t.y1d) return CLS_EMPTY;
     const float LIM = 4194304.0f;
-    bool finite = fabsf(t.x1) < LIM && fabsf(t.y1) < LIM && fabsf(t.x2) < LIM && fabsf(t.y2) < LIM &&
-                  fabsf(t.x3) < LIM && fabsf(t.y3) < LIM;
-    if (!finite) return CLS_FULL;
+    // all six |coordinates| < 2^22 and none NaN: the IEEE 754-2019 maximum
+    // propagates NaN, so one compare of the largest decides
+    const float mx = __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(t.x1), fabsf(t.x2)), fabsf(t.x3));
+    const float my = __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(t.y1), fabsf(t.y2)), fabsf(t.y3));
+    if (!(__builtin_elementwise_maximum(mx, my) < LIM)) return CLS_FULL;
     const float K = 8.0f * 5.9604645e-08f;
     float X0 = (float)t.x0, X1 = (float)(t.x1d - 1), Y0 = (float)t.y0, Y1 = (float)(t.y1d - 1);
     float dx0 = fmaxf(fabsf(X0 - t.x2), fabsf(X1 - t.x2)), dy0 = fmaxf(fabsf(Y0 - t.y2), fabsf(Y1 - t.y2));

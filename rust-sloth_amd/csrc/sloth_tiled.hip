@@ -157,18 +157,29 @@ __device__ __forceinline__ void put_entry(uint64_t gt, uint32_t pos, uint32_t en
 // ---------------------------------------------------------------------------
 constexpr int BIN_WIN = 16;
 
-// Wavefront-wide min / max (all 64 lanes must be active).  A block reduces
-// through these first so that each wavefront issues one LDS atomic: LDS
-// serialises lanes that hit the same address.
+// Wavefront-wide min / max (all 64 lanes must be active), returned uniform.
+// A block reduces through these first so that each wavefront issues one LDS
+// atomic: LDS serialises lanes that hit the same address.  Within each row of
+// 16 lanes a DPP butterfly (quad_perm [1,0,3,2] and [2,3,0,1], then row_ror 4
+// and 8) leaves the row's extreme in every lane; the four rows are then read
+// by readlane.  (Shuffles through ds_bpermute cost six LDS round trips.)
+template <class Op>
+__device__ __forceinline__ int wave_reduce(int v, Op op)
+{
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x124, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false));
+    return op(op(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+              op(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
 __device__ __forceinline__ int wave_min(int v)
 {
-    for (int sh = 32; sh >= 1; sh >>= 1) v = min(v, __shfl_xor(v, sh));
-    return v;
+    return wave_reduce(v, [](int a, int b) { return min(a, b); });
 }
 __device__ __forceinline__ int wave_max(int v)
 {
-    for (int sh = 32; sh >= 1; sh >>= 1) v = max(v, __shfl_xor(v, sh));
-    return v;
+    return wave_reduce(v, [](int a, int b) { return max(a, b); });
 }   // the block aggregates appends in a 16 x 16 window of tiles
 
 // Grid: x = blocks of one draw's triangles, y = draw index - d_lo, so the draw,
