@@ -247,7 +247,7 @@ struct Job {
             }
             fr.draw_end = (uint32_t)k;
             fr.n_tris = tri;
-            if (tri >= (1u << 30)) small_frames = false;
+            if (tri >= TILED_MAX_TRIS) small_frames = false;
             uint64_t WH = (uint64_t)fr.W * fr.H;
             cells += WH + (fr.image ? fr.H : 0);
             // frames start on 256-byte boundaries: a frame's rows of tile
@@ -264,7 +264,7 @@ struct Job {
         n_rows = rows;
         n_items = items;
         // the tiled path keeps a draw's shader index in 8 bits of an LDS word
-        // and flags in the two top bits of a 32-bit tile-list entry
+        // and a triangle index with a cache row in a 32-bit hash word
         path = (all_even && small_frames && ramps.size() <= 256 && !force_global()) ? PATH_TILED : PATH_GLOBAL;
         int rc;
         if ((rc = d_frames.reserve(sizeof(Frame) * std::max<size_t>(1, frames.size())))) return rc;

@@ -183,16 +183,16 @@ __device__ __forceinline__ uint8_t ramp_glyph(const Ramp& r, float s)
         // Non-decreasing thresholds: {k : thr[k] < s} is a prefix [0, i), so
         // i is the first index with s <= thr[i] (none when i == n); a NaN
         // shade passes no threshold.  The host pads thr[n..16) with +inf,
-        // which no shade exceeds, so i is found by a fixed binary search over
-        // all 16 entries (steps 8, 4, 2, 1 reach 15; the last step takes
-        // i = 15 to 16) instead of a divergent scan.
-        static_assert(MAX_RAMP == 16, "the search below covers 16 thresholds");
-        uint32_t i = 0;
-        i += (r.thr[i + 7] < s) ? 8u : 0u;
-        i += (r.thr[i + 3] < s) ? 4u : 0u;
-        i += (r.thr[i + 1] < s) ? 2u : 0u;
-        i += (r.thr[i] < s) ? 1u : 0u;
-        i += (r.thr[i] < s) ? 1u : 0u;
+        // which no shade exceeds, so i is counted over all 16 entries in two
+        // steps of independent reads: c blocks of four lie wholly below s
+        // (their last entries are), and the block b = min(c, 3) adds its own
+        // entries below s (four when c = 4, giving 16).
+        static_assert(MAX_RAMP == 16, "the count below covers 16 thresholds");
+        const uint32_t c = (uint32_t)(r.thr[3] < s) + (uint32_t)(r.thr[7] < s) + (uint32_t)(r.thr[11] < s) +
+                           (uint32_t)(r.thr[15] < s);
+        const uint32_t b = 4u * min(c, 3u);
+        const uint32_t i = b + (uint32_t)(r.thr[b] < s) + (uint32_t)(r.thr[b + 1] < s) + (uint32_t)(r.thr[b + 2] < s) +
+                           (uint32_t)(r.thr[b + 3] < s);
         return (i >= r.n || s != s) ? r.fallback : r.glyph[i];
     }
     for (uint32_t i = 0; i < r.n; ++i)

@@ -42,6 +42,9 @@
 namespace sloth {
 
 constexpr uint32_t MAX_GROUPS = 8;        // frame groups of a pipelined launch
+// Frames on the tiled path hold fewer triangles than this: the raster's hash
+// table keeps a triangle index and its 8-bit cache row in one 32-bit word.
+constexpr uint32_t TILED_MAX_TRIS = (1u << 24) - 1;
 constexpr uint32_t PIPE_MIN_FRAMES = 16;  // frames per group at least
 
 struct TiledPlan {

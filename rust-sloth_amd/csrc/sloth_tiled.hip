@@ -40,7 +40,7 @@ __global__ void k_tile_clear(uint32_t* __restrict__ cursor, uint64_t n_tiles, in
 // the index of a CLS_BOX one (the raster recomputes either box from the
 // triangle's x extent), or CAREFUL_FLAG | row of the careful-box table for
 // triangles whose box k_careful computed.  Frames on this path hold fewer than
-// 2^30 triangles.
+// TILED_MAX_TRIS triangles.
 constexpr uint32_t CAREFUL_FLAG = 0x80000000u;
 constexpr uint32_t LOOSE_FLAG = 0x40000000u;
 constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
@@ -372,16 +372,10 @@ __device__ __forceinline__ void careful_one(const Draw* __restrict__ draws, cons
                 last = max(last, y);
             }
         }
-        for (int sh = 32; sh >= 1; sh >>= 1) {
-            lo_x = min(lo_x, __shfl_xor(lo_x, sh));
-            hi_x = max(hi_x, __shfl_xor(hi_x, sh));
-            first = min(first, __shfl_xor(first, sh));
-            last = max(last, __shfl_xor(last, sh));
-        }
-        bx0 = lo_x;
-        bx1 = hi_x;
-        by0 = first;
-        by1 = last;
+        bx0 = wave_min(lo_x);
+        bx1 = wave_max(hi_x);
+        by0 = wave_min(first);
+        by1 = wave_max(last);
     } else if (cls != CLS_FULL) {
         bx1 = -1;   // decided after all (cannot happen for entries k_bin sent here)
     }
@@ -437,7 +431,7 @@ __global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws,
 // k_tile_raster
 // ---------------------------------------------------------------------------
 constexpr uint32_t CHUNK = RASTER_BLOCK;   // list entries whose triangles sit in LDS at once (one per thread)
-constexpr uint32_t HASH_CAP = 512;         // open-addressing table: triangle -> cache row
+constexpr uint32_t HASH_CAP = 512;         // open-addressing table: triangle << 8 | cache row
 constexpr uint32_t CACHE_WORDS = 12;
 constexpr uint32_t LDS_RAMPS = 4;          // shaders staged in LDS per tile
 constexpr uint32_t SPAN_MIN_W = 6;         // box width (pixels) from which a wavefront walks row spans
@@ -654,9 +648,8 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             return ramp_glyph(ramps[r], sh);
         };
         uint64_t* keys = smem;
-        uint32_t* htag = (uint32_t*)(keys + S);
+        uint32_t* htag = (uint32_t*)(keys + S);   // T << 8 | cache row (< CHUNK), or EMPTY32
         float* cache = (float*)(htag + HASH_CAP);
-        uint8_t* hrow = (uint8_t*)(cache + CHUNK * CACHE_WORDS);   // cache row (< CHUNK) of a hash slot
         const bool one_draw = fr.draw_end - fr.draw_begin == 1;
         const bool multi = n > CHUNK;
 
@@ -667,11 +660,11 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             uint32_t h = hash_tri(T), tag;
             for (uint32_t probe = 0; probe < HASH_CAP; ++probe) {
                 tag = htag[h];
-                if (tag == T || tag == EMPTY32) break;
+                if ((tag >> 8) == T || tag == EMPTY32) break;
                 h = (h + 1) & (HASH_CAP - 1);
             }
-            if (tag != T) return false;
-            const float4* row4 = (const float4*)(cache + hrow[h] * CACHE_WORDS);
+            if (tag == EMPTY32 || (tag >> 8) != T) return false;
+            const float4* row4 = (const float4*)(cache + (tag & 0xFFu) * CACHE_WORDS);
             const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
             const int col = my_col;
             const float px = (float)(sub ? (col >> 1) : ((col + g.W) >> 1));
@@ -713,10 +706,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 row4[2] = make_float4(t.z3 - t.z1, t.a, na, __uint_as_float(cw));
                 uint32_t h = hash_tri(T);
                 for (uint32_t probe = 0; probe < HASH_CAP; ++probe) {
-                    if (atomicCAS(&htag[h], EMPTY32, T) == EMPTY32) {
-                        hrow[h] = (uint8_t)p;
-                        break;
-                    }
+                    if (atomicCAS(&htag[h], EMPTY32, (T << 8) | p) == EMPTY32) break;
                     h = (h + 1) & (HASH_CAP - 1);
                 }
                 RowSpans spans;
@@ -773,7 +763,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 while (m) {
                     const int l = __builtin_ctzll(m);
                     m &= m - 1;
-                    const uint32_t T = __shfl(big_T, l), rc = __shfl(big_rect, l);
+                    const uint32_t T = __builtin_amdgcn_readlane(big_T, l), rc = __builtin_amdgcn_readlane(big_rect, l);
                     const float4* row4 = (const float4*)(cache + ((threadIdx.x & ~63u) + (uint32_t)l) * CACHE_WORDS);
                     const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
                     const float x1 = q0.x, y1 = q0.y, x2 = q0.z, y2 = q0.w, x3 = q1.x, y3 = q1.y;
@@ -879,9 +869,9 @@ int TiledPlan::set_tile(uint32_t tw, uint32_t th)
 
 size_t TiledPlan::raster_lds() const
 {
-    // keys (8 B per slot), hash tags (4 B) and rows (1 B), triangle rows
+    // keys (8 B per slot), hash words (4 B), triangle rows
     size_t S = (size_t)(TW / 2) * TH;
-    return S * 8 + (size_t)HASH_CAP * 5 + (size_t)CHUNK * CACHE_WORDS * 4;
+    return S * 8 + (size_t)HASH_CAP * 4 + (size_t)CHUNK * CACHE_WORDS * 4;
 }
 
 int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
