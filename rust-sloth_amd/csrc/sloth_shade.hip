@@ -139,20 +139,21 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_fwd(ShadeArgs p)
     block_partials(loss, 0.0, 0.0, 0.0, p.partial, blockIdx.x);
 }
 
-#ifndef SLOTH_BWD_TILE_H
-#define SLOTH_BWD_TILE_H 16
-#endif
-constexpr uint32_t BWD_TILE_W = 64, BWD_TILE_H = SLOTH_BWD_TILE_H;   // z entries per backward tile
+constexpr uint32_t BWD_TILE_W = 64, BWD_TILE_H = 32;   // z entries per backward tile
 constexpr uint32_t BWD_TABLE = 256;     // LDS rows for the tile's distinct vertices
 constexpr uint32_t BWD_PROBES = 32;     // a fragment that finds no row adds to HBM directly
 constexpr uint32_t BWD_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t BWD_RUN = 8;         // consecutive z entries of a row per thread
+static_assert(BWD_TILE_W % BWD_RUN == 0 && SHADE_BLOCK / (BWD_TILE_W / BWD_RUN) == BWD_TILE_H,
+              "the block's runs cover one tile in one pass");
 
 // Backward: a block takes 64 x BWD_TILE_H tiles of z entries (tile j of the
 // job; frames hold tile_begin[f] .. tile_begin[f+1]), grid-striding over the
-// tiles.  A fragment's 6 colour / normal terms for each of its three vertices
-// go into an LDS table keyed by vertex; after the tile, each table row adds
-// its sums to the vertex's binary64 accumulators with one global atomic per
-// term.  Several pixels of a tile share each vertex, so this takes most
+// tiles, each thread a run of BWD_RUN entries of a row.  A fragment's 6
+// colour / normal terms for each of its three vertices are summed over the
+// run's consecutive fragments of one triangle, then go into an LDS table
+// keyed by vertex; after the tile, each table row adds its sums to the
+// vertex's binary64 accumulators with one global atomic per term.  Several pixels of a tile share each vertex, so this takes most
 // same-address atomics off the memory system.  A vertex that finds no free
 // row adds its terms to the accumulators directly.  The loss and the light
 // gradient are summed per block.
@@ -175,13 +176,51 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_bwd(ShadeArgs p, const ui
         const uint32_t tiles_x = (fr.W + BWD_TILE_W - 1) / BWD_TILE_W;
         const uint32_t local = (uint32_t)(j - tile_begin[lo]);
         const uint32_t ty = local / tiles_x, tx = local - ty * tiles_x;
-        const uint32_t col = tx * BWD_TILE_W + (threadIdx.x % BWD_TILE_W);
-        for (uint32_t row = ty * BWD_TILE_H + (threadIdx.x / BWD_TILE_W); row < min(fr.H, (ty + 1) * BWD_TILE_H);
-             row += SHADE_BLOCK / BWD_TILE_W) {
-            if (col >= fr.W) break;
-            const uint64_t g = fr.z_off + (uint64_t)row * fr.W + col;
-            const int32_t v = p.vis[g];
-            if (v >= 0) {
+        // Each thread takes BWD_RUN consecutive z entries of a row.  Their
+        // terms are summed in registers while they share a triangle and go
+        // to the table when the triangle changes or the run ends: adjacent
+        // pixels mostly share a triangle, and lanes that add to the same
+        // LDS row in one instruction are serialised.
+        constexpr uint32_t RUNS = BWD_TILE_W / BWD_RUN;   // threads per tile row
+        const uint32_t col0 = tx * BWD_TILE_W + (threadIdx.x % RUNS) * BWD_RUN;
+        const float L0 = p.light[0], L1 = p.light[1], L2 = p.light[2];
+        int32_t cur = -1;   // triangle of the sums below, or none
+        uint32_t cvid[3] = {0, 0, 0};
+        float acc[3][6];
+        // adds the run's sums for triangle `cur` to the table
+        auto flush = [&]() {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t vid = cvid[k];
+                uint32_t h = (vid * 0x9E3779B1u) >> 24;   // BWD_TABLE = 256 rows
+                bool found = false;
+                for (uint32_t probe = 0; probe < BWD_PROBES; ++probe) {
+                    const uint32_t old = atomicCAS(&s_key[h], BWD_EMPTY, vid);
+                    if (old == BWD_EMPTY || old == vid) {
+                        found = true;
+                        break;
+                    }
+                    h = (h + 1) & (BWD_TABLE - 1);
+                }
+                if (found) {
+                    float* dst = s_acc + h * 6;
+#pragma unroll
+                    for (int m = 0; m < 6; ++m) atomicAdd(dst + m, acc[k][m]);
+                } else {
+                    double* dst = p.vert_acc + (size_t)vid * 6;
+#pragma unroll
+                    for (int m = 0; m < 6; ++m) unsafeAtomicAdd(dst + m, (double)acc[k][m]);
+                }
+            }
+        };
+        for (uint32_t row = ty * BWD_TILE_H + threadIdx.x / RUNS; row < min(fr.H, (ty + 1) * BWD_TILE_H);
+             row += SHADE_BLOCK / RUNS) {
+            for (uint32_t j = 0; j < BWD_RUN; ++j) {
+                const uint32_t col = col0 + j;
+                if (col >= fr.W) break;
+                const uint64_t g = fr.z_off + (uint64_t)row * fr.W + col;
+                const int32_t v = p.vis[g];
+                if (v < 0) continue;
                 const Fragment f = shade_fragment(p, fr, g, v);
                 const float o0 = f.I * f.c0, o1 = f.I * f.c1, o2 = f.I * f.c2;
                 const float r0 = o0 - p.target[g * 3 + 0];
@@ -191,39 +230,37 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_bwd(ShadeArgs p, const ui
                 const float g0 = 2.f * r0, g1 = 2.f * r1, g2 = 2.f * r2;
                 const float dI = g0 * f.c0 + g1 * f.c1 + g2 * f.c2;
                 const float* G = f.G;
-                const float L0 = p.light[0], L1 = p.light[1], L2 = p.light[2];
                 // G^T L (model-space direction of the light for this view)
                 const float gl0 = G[0] * L0 + G[3] * L1 + G[6] * L2;
                 const float gl1 = G[1] * L0 + G[4] * L1 + G[7] * L2;
                 const float gl2 = G[2] * L0 + G[5] * L1 + G[8] * L2;
+                const int32_t T = v >> 1;
+                if (T != cur) {
+                    if (cur >= 0) flush();
+                    cur = T;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        cvid[k] = f.vid[k];
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) acc[k][m] = 0.0f;
+                    }
+                }
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const float lk = f.lam[k];
-                    const float terms[6] = {lk * f.I * g0, lk * f.I * g1, lk * f.I * g2,
-                                            dI * lk * gl0, dI * lk * gl1, dI * lk * gl2};
-                    const uint32_t vid = f.vid[k];
-                    uint32_t h = (vid * 0x9E3779B1u) >> 24;   // BWD_TABLE = 256 rows
-                    bool found = false;
-                    for (uint32_t probe = 0; probe < BWD_PROBES; ++probe) {
-                        const uint32_t old = atomicCAS(&s_key[h], BWD_EMPTY, vid);
-                        if (old == BWD_EMPTY || old == vid) {
-                            found = true;
-                            break;
-                        }
-                        h = (h + 1) & (BWD_TABLE - 1);
-                    }
-                    if (found) {
-                        float* acc = s_acc + h * 6;
-                        for (int m = 0; m < 6; ++m) atomicAdd(acc + m, terms[m]);
-                    } else {
-                        double* dst = p.vert_acc + (size_t)vid * 6;
-                        for (int m = 0; m < 6; ++m) unsafeAtomicAdd(dst + m, (double)terms[m]);
-                    }
+                    acc[k][0] += lk * f.I * g0;
+                    acc[k][1] += lk * f.I * g1;
+                    acc[k][2] += lk * f.I * g2;
+                    acc[k][3] += dI * lk * gl0;
+                    acc[k][4] += dI * lk * gl1;
+                    acc[k][5] += dI * lk * gl2;
                 }
                 dl0 += (double)(dI * f.n0);
                 dl1 += (double)(dI * f.n1);
                 dl2 += (double)(dI * f.n2);
             }
+            if (cur >= 0) flush();
+            cur = -1;
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < BWD_TABLE; i += SHADE_BLOCK) {
