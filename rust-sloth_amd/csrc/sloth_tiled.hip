@@ -243,6 +243,23 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     const bool has_direct = r.dx0 <= r.dx1 && r.dy0 <= r.dy1;
     const bool has_wrap = r.wx0 <= r.wx1 && r.wy0 <= r.wy1;
     const bool small = has_direct && !has_wrap && r.dx1 - r.dx0 <= 1 && r.dy1 - r.dy0 <= 1;
+    const bool wide = r.dx1 > r.dx0, tall = r.dy1 > r.dy0;
+
+    // The pool ranges of a small box's (up to 2 x 2) tiles, loaded now so
+    // that the loads are in flight through the barriers and atomics below
+    // rather than after them.  A row of two tiles needs three offsets.
+    uint64_t o00 = 0, o01 = 0, o02 = 0, o10 = 0, o11 = 0, o12 = 0;
+    if (!COUNT && small) {
+        const uint64_t g0 = tile_begin + (uint64_t)r.dy0 * tiles_x + (uint64_t)r.dx0;
+        o00 = off[g0];
+        o01 = off[g0 + 1];
+        if (wide) o02 = off[g0 + 2];
+        if (tall) {
+            o10 = off[g0 + tiles_x];
+            o11 = off[g0 + tiles_x + 1];
+            if (wide) o12 = off[g0 + tiles_x + 2];
+        }
+    }
 
     w_cnt[threadIdx.x] = 0;
     s_nl[threadIdx.x] = -1;
@@ -282,7 +299,6 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 
     const int ox = s_tx0, oy = s_ty0;
     const bool inwin = small && r.dx1 - ox < BIN_WIN && r.dy1 - oy < BIN_WIN;
-    const bool wide = r.dx1 > r.dx0, tall = r.dy1 > r.dy0;
     const int c = (r.dy0 - oy) * BIN_WIN + (r.dx0 - ox);
     uint32_t lp0 = 0, lp1 = 0, lp2 = 0, lp3 = 0;
     if (inwin) {
@@ -319,12 +335,17 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     __syncthreads();
 
     if (inwin) {
-        const uint64_t g0 = tile_begin + (uint64_t)r.dy0 * tiles_x + (uint64_t)r.dx0;
-        put_entry<COUNT>(g0, w_base[c] + lp0, entry, off, pool, overflow, frame);
-        if (wide) put_entry<COUNT>(g0 + 1, w_base[c + 1] + lp1, entry, off, pool, overflow, frame);
-        if (tall) put_entry<COUNT>(g0 + tiles_x, w_base[c + BIN_WIN] + lp2, entry, off, pool, overflow, frame);
-        if (wide && tall)
-            put_entry<COUNT>(g0 + tiles_x + 1, w_base[c + BIN_WIN + 1] + lp3, entry, off, pool, overflow, frame);
+        if (!COUNT) {
+            // tile (dx0 + i, dy0 + j) holds pool[o_ji, o_j(i+1))
+            auto put = [&](uint64_t o, uint64_t e, uint32_t pos) {
+                if (pos < e - o) pool[o + pos] = entry;
+                else overflow[frame] = 1u;
+            };
+            put(o00, o01, w_base[c] + lp0);
+            if (wide) put(o01, o02, w_base[c + 1] + lp1);
+            if (tall) put(o10, o11, w_base[c + BIN_WIN] + lp2);
+            if (wide && tall) put(o11, o12, w_base[c + BIN_WIN + 1] + lp3);
+        }
     } else if (has_direct || has_wrap) {
         // large or wrapping boxes: one global atomic per tile
         TileIter it;
