@@ -59,8 +59,8 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
-        print(json.dumps({"error": "this build runs on one GPU; frames shard across GPUs without "
-                          "exchange (see DESIGN.md), but multi-GPU runs are not part of it"}))
+        print(json.dumps({"error": "one GPU only: the frames of a batch are independent and would shard "
+                          "across GPUs without any exchange, but multi-GPU runs are not part of this library"}))
         return 2
 
     import sloth_amd
