@@ -14,7 +14,8 @@ Forms timed:
   plain    one launch sequence per frame, device sync after each
   cpu      the oracle (C restatement of the reference, 1 thread) on a bounded
            sample of frames; the same frames are compared against the GPU
-           output bit for bit
+           output bit for bit.  Also frame-parallel over the host's cores
+           (cpu_baseline.parallel), frames being independent
 
 Prints ONE JSON line.  Torch is not imported: it bundles a second HIP runtime.
 """
@@ -149,6 +150,18 @@ def main() -> int:
         fr = oracle.Frame(W, H, True)
         fr.draw_mesh(U, m.xyz, m.rgb, Rs[0])
         frags = int(fr.stats[1])
+        # the same C path frame-parallel over the host's cores (frames are
+        # independent): the fairer CPU comparison, reported beside the
+        # single-thread baseline
+        ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        ncpu = max(1, min(ncpu, 16))
+        idx2 = np.linspace(0, F - 1, 2 * ncpu).round().astype(int)
+        log(f"bench: CPU baseline frame-parallel on {ncpu} threads, {len(idx2)} frames")
+        t0 = time.perf_counter()
+        oracle.render_frames(W, H, U, Rs[idx2], m.xyz, m.rgb, image=True, nthreads=ncpu, keep=False)
+        par_s = time.perf_counter() - t0
+        cpu["parallel"] = {"value": round(len(idx2) / par_s, 4), "cores": ncpu,
+                           "sample": f"{len(idx2)} sweep frames, one per thread at a time, {par_s:.1f} s"}
 
     extra = {}
     if not args.no_extra:
