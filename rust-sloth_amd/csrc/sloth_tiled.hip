@@ -586,8 +586,12 @@ __device__ __forceinline__ uint32_t entry_tri(uint32_t e, const uint4* __restric
 // wavefront takes its lanes' big boxes one at a time and all 64 lanes stride
 // over the box.  The variant costs registers on every path, so it is launched
 // only for lists with many big boxes (TiledPlan::big_walk).
+//
+// ONE: every frame of the launch has exactly one draw, so the draw is the
+// same for the whole workgroup and its fields come through scalar loads that
+// do not wait for the list entries.
 // Tile (tx, ty) of frame fr, whose list is pool[o, o + n).
-template <bool BIG>
+template <bool BIG, bool ONE>
 __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, const Ramp* __restrict__ ramps,
                                             const Ramp* s_ramp, const Frame& fr, uint32_t tx, uint32_t ty, uint64_t o,
                                             uint32_t n, int tws, int ths, const uint32_t* __restrict__ pool,
@@ -714,7 +718,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 int bx0, bx1, by0, by1;
                 bool given, tight;
                 const uint32_t T = entry_tri(list[base + p], careful_box, bx0, bx1, by0, by1, given, tight);
-                const Draw& dr = one_draw ? draws[fr.draw_begin] : draw_of_t(draws, fr, T);
+                const Draw& dr = ONE ? draws[fr.draw_begin] : one_draw ? draws[fr.draw_begin] : draw_of_t(draws, fr, T);
                 const uint32_t li = T - dr.tri_begin;
                 Tri t;
                 setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
@@ -844,7 +848,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
 // the compiler gives the plain instance 76 VGPRs (6 wavefronts per SIMD);
 // held to 7 it fits 72 with SGPR spills to VGPR lanes only, and runs ~3%
 // faster.  The big-box instance is faster at its own 79 (6 per SIMD).
-template <bool BIG>
+template <bool BIG, bool ONE>
 __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_eu(BIG ? 6 : 7, 8))) k_tile_raster(
     const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
     int tws, int ths, const uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
@@ -866,7 +870,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_
     if (n > 0)
         for (uint32_t i = threadIdx.x; i < min(n_ramps, LDS_RAMPS) * (uint32_t)sizeof(Ramp) / 4; i += blockDim.x)
             ((uint32_t*)s_ramp)[i] = ((const uint32_t*)ramps)[i];
-    raster_tile<BIG>(draws, ramps, s_ramp, fr, tx, ty, o, n, tws, ths, pool, nl, careful_box, cells, zbuf, win, win_raw,
+    raster_tile<BIG, ONE>(draws, ramps, s_ramp, fr, tx, ty, o, n, tws, ths, pool, nl, careful_box, cells, zbuf, win, win_raw,
                 smem);
 }
 
@@ -1043,7 +1047,11 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
             my = std::max(my, frames[f].tiles_y);
         }
         if (mx == 0 || my == 0) continue;
-        hipLaunchKernelGGL(big_walk ? k_tile_raster<true> : k_tile_raster<false>, dim3(mx, my, f1 - f0),
+        bool one = true;   // every frame of the range has exactly one draw
+        for (uint32_t f = f0; f < f1 && one; ++f) one = frames[f].draw_end - frames[f].draw_begin == 1;
+        const auto kern = big_walk ? (one ? k_tile_raster<true, true> : k_tile_raster<true, false>)
+                                   : (one ? k_tile_raster<false, true> : k_tile_raster<false, false>);
+        hipLaunchKernelGGL(kern, dim3(mx, my, f1 - f0),
                            dim3(RASTER_BLOCK), raster_lds(), s, d_draws, d_ramps, d_frames, f0, log2u(TW), log2u(TH),
                            d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
                            d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps);
