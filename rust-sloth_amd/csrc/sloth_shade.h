@@ -25,12 +25,12 @@ struct ShadeArgs {
     double* vert_acc;         // [n_verts][6]: d color, d normal (binary64 sums)
     uint64_t n_slots;
     uint32_t n_tris, n_verts;
-    const uint64_t* bwd_tile_begin;   // per frame (+1): first backward tile (64 x 4 z entries)
+    const uint64_t* bwd_tile_begin;   // per frame (+1): first backward tile
     uint64_t n_bwd_tiles;
 };
 
 uint32_t shade_blocks(uint64_t n_slots);
-// Backward tiles (64 x 4 z entries) per frame, prefix-summed: begin[0..n_frames]
+// Backward tiles per frame, prefix-summed: begin[0..n_frames]
 void bwd_tiles(const Frame* frames, uint32_t n_frames, uint64_t* begin);
 int launch_shade(const ShadeArgs& a, bool backward, hipStream_t s);
 // binary64 vertex sums -> binary32 gradients
