@@ -110,23 +110,18 @@ def main() -> int:
     if dominant:
         t = prof[dominant] / 1e3
         achieved = alg_bytes / t / 1e9
-        traffic = valu_ms = None
+        traffic = None
         # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
-        # summary of this kernel, and the time its VALU instruction count
-        # (SQ_INSTS_VALU) takes at the issue rate of 1024 SIMDs
+        # summary of this kernel
         tf = os.path.join(ROOT, "profiles", f"{dominant}_sweep_1080p.md")
         if os.path.exists(tf):
             with open(tf) as fh:
                 for line in fh:
                     if line.startswith("hbm_bytes_per_launch:"):
                         traffic = int(line.split(":")[1])
-                    elif line.startswith("valu_issue_ms:"):
-                        valu_ms = float(line.split(":")[1])
         roof = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "alg_bytes_per_launch": alg_bytes, "kernel_ms": round(prof[dominant], 4),
-                "valu_issue_ms": valu_ms,
-                "valu_issue_frac": round(valu_ms / prof[dominant], 4) if valu_ms else None,
                 "stages_ms": {k: round(v, 4) for k, v in prof.items()}}
 
     cpu = None
