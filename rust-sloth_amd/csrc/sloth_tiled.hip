@@ -845,11 +845,12 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
 // One workgroup per tile; grid (tile x, tile y, frame - f_lo).  Blocks past a
 // smaller frame's tile grid return at once.
 // Occupancy: LDS allows 7 workgroups of 4 wavefronts per CU.  Left to itself
-// the compiler gives the plain instance 76 VGPRs (6 wavefronts per SIMD);
-// held to 7 it fits 72 with SGPR spills to VGPR lanes only, and runs ~3%
-// faster.  The big-box instance is faster at its own 79 (6 per SIMD).
+// the compiler gives the instances 76-80 VGPRs (6 wavefronts per SIMD); held
+// to 7 they fit 72 with SGPR spills to VGPR lanes only, and run ~3% faster.
+// The big-box instance for several draws per frame would spill VGPRs to
+// scratch at 72 and keeps 6.
 template <bool BIG, bool ONE>
-__global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_eu(BIG ? 6 : 7, 8))) k_tile_raster(
+__global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_eu(BIG && !ONE ? 6 : 7, 8))) k_tile_raster(
     const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
     int tws, int ths, const uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ pool, const int32_t* __restrict__ nl, const uint4* __restrict__ careful_box,
