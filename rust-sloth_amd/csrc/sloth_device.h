@@ -254,15 +254,20 @@ enum : uint32_t { CLS_EMPTY = 0, CLS_BOX = 1, CLS_SPANS = 2, CLS_FULL = 3, CLS_T
 // +-2^22, fall to CLS_SPANS (exact per-row spans found from the monotonicity of
 // each FP32 edge function along a row) or CLS_FULL (every pixel of the
 // domain).  u = 2^-24; the factor 8 covers the bound's own rounding.
+//
+// Every test is evaluated and the class picked by selects, lowest precedence
+// first (no branches: a wavefront's lanes take different classes).  Where a
+// higher test decides, the values of the lower ones (NaN or inf included) are
+// not used.
 __device__ __forceinline__ uint32_t setup_class(const Tri& t)
 {
-    if (t.x0 >= t.x1d || t.y0 >= t.y1d) return CLS_EMPTY;
+    const bool no_pixels = t.x0 >= t.x1d || t.y0 >= t.y1d;
     const float LIM = 4194304.0f;
     // all six |coordinates| < 2^22 and none NaN: the IEEE 754-2019 maximum
     // propagates NaN, so one compare of the largest decides
     const float mx = __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(t.x1), fabsf(t.x2)), fabsf(t.x3));
     const float my = __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(t.y1), fabsf(t.y2)), fabsf(t.y3));
-    if (!(__builtin_elementwise_maximum(mx, my) < LIM)) return CLS_FULL;
+    const bool finite = __builtin_elementwise_maximum(mx, my) < LIM;
     const float K = 8.0f * 5.9604645e-08f;
     float X0 = (float)t.x0, X1 = (float)(t.x1d - 1), Y0 = (float)t.y0, Y1 = (float)(t.y1d - 1);
     float dx0 = fmaxf(fabsf(X0 - t.x2), fabsf(X1 - t.x2)), dy0 = fmaxf(fabsf(Y0 - t.y2), fabsf(Y1 - t.y2));
@@ -274,14 +279,14 @@ __device__ __forceinline__ uint32_t setup_class(const Tri& t)
     float ey = t.y3 - t.y1, ex = t.x3 - t.x1;
     float o = t.A2 * ey - t.B2 * ex;
     float eo = K * (fabsf(t.A2) * fabsf(ey) + fabsf(t.B2) * fabsf(ex));
-    if (o + eo < -bound) return CLS_EMPTY;          // back face, provably uncovered
-    float lo = o - eo;
-    if (lo > 0.0f) {
-        const float bw = bound * (t.maxx - t.minx);
-        if (bw < 0.0078125f * lo) return CLS_TIGHT;
-        if (bw < 0.5f * lo) return CLS_BOX;
-    }
-    return CLS_SPANS;
+    const float lo = o - eo;
+    const float bw = bound * (t.maxx - t.minx);
+    uint32_t c = CLS_SPANS;
+    c = lo > 0.0f && bw < 0.5f * lo ? CLS_BOX : c;
+    c = lo > 0.0f && bw < 0.0078125f * lo ? CLS_TIGHT : c;
+    c = o + eo < -bound ? CLS_EMPTY : c;           // back face, provably uncovered
+    c = finite ? c : CLS_FULL;
+    return no_pixels ? CLS_EMPTY : c;
 }
 
 // Per-row candidate spans of a CLS_BOX / CLS_TIGHT triangle, for walking a

@@ -219,27 +219,21 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     const uint64_t tile_begin = fr.tile_begin;
     const uint32_t tiles_x = fr.tiles_x;
 
-    int y0 = 0, y1d = 0;
-    bool careful = false, big = false;
-    uint32_t entry = tri;
+    // Lanes past the draw's end set up its last triangle and discard it, so
+    // that the set-up below has no branch around it (the lanes of a wavefront
+    // take different classes; selects cost less than divergent paths).
+    Tri t;
+    setup_tri(dr.M, dr.xyz + (size_t)(valid ? local : dr.n - 1) * 9, W, H, t);
+    const uint32_t cls = valid ? setup_class(t) : CLS_EMPTY;
+    const bool boxed = cls == CLS_BOX || cls == CLS_TIGHT;
+    const bool careful = cls == CLS_SPANS || cls == CLS_FULL;
+    const uint32_t entry = tri | (cls == CLS_BOX ? LOOSE_FLAG : 0u);
+    const int y0 = valid ? t.y0 : 0, y1d = valid ? t.y1d : 0;
+    int bx0 = 1, bx1 = 0;
+    if (boxed) box_x(t, cls == CLS_TIGHT, bx0, bx1);
     TileRects r;
-    empty_rects(r);
-    if (valid) {
-        Tri t;
-        setup_tri(dr.M, dr.xyz + (size_t)local * 9, W, H, t);
-        y0 = t.y0;
-        y1d = t.y1d;
-        const uint32_t cls = setup_class(t);
-        if (cls == CLS_BOX || cls == CLS_TIGHT) {
-            int bx0, bx1;
-            box_x(t, cls == CLS_TIGHT, bx0, bx1);
-            if (cls == CLS_BOX) entry |= LOOSE_FLAG;
-            if (bx0 <= bx1) tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);
-            if (COUNT) big = (int64_t)(bx1 - bx0 + 1) * (int64_t)(t.y1d - t.y0) > (int64_t)BIG_BOX;
-        } else if (cls != CLS_EMPTY) {
-            careful = true;
-        }
-    }
+    tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);   // empty when bx0 > bx1
+    const bool big = COUNT && boxed && (int64_t)(bx1 - bx0 + 1) * (int64_t)(t.y1d - t.y0) > (int64_t)BIG_BOX;
     const bool has_direct = r.dx0 <= r.dx1 && r.dy0 <= r.dy1;
     const bool has_wrap = r.wx0 <= r.wx1 && r.wy0 <= r.wy1;
     const bool small = has_direct && !has_wrap && r.dx1 - r.dx0 <= 1 && r.dy1 - r.dy0 <= 1;
