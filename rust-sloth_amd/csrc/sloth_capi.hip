@@ -330,8 +330,6 @@ struct Job {
         uint32_t d_lo = a.draw_begin, d_hi = b.draw_end;
         uint64_t it_lo = d_lo < draws.size() ? draws[d_lo].item_begin : n_items;
         uint64_t it_hi = d_hi < draws.size() ? draws[d_hi].item_begin : n_items;
-        uint64_t c_lo = a.cell_begin;
-        uint64_t c_hi = b.cell_begin + frame_cells(b);
         if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
         {
             uint64_t n = std::max(z_hi - z_lo, r_hi - r_lo);
@@ -348,14 +346,29 @@ struct Job {
             HIP_TRY(hipGetLastError());
         }
         if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
-        if (c_hi > c_lo) {
-            hipLaunchKernelGGL(k_resolve_global, dim3(blocks_for(c_hi - c_lo)), dim3(256), 0, s, d_draws.as<Draw>(),
-                               d_ramps.as<Ramp>(), d_frames.as<Frame>(), f_lo, f_hi, c_lo, c_hi - c_lo,
-                               d_keys.as<uint64_t>(), d_nl.as<int32_t>(), d_cells.as<uint32_t>(), d_z.as<float>(), win,
+        // grids (cells of the largest frame / 256, frames), at most 65535
+        // frames per launch
+        uint64_t max_wh = 0, max_cells = 0;
+        for (uint32_t f = f_lo; f < f_hi; ++f) {
+            max_wh = std::max<uint64_t>(max_wh, (uint64_t)frames[f].W * frames[f].H);
+            max_cells = std::max<uint64_t>(max_cells, frame_cells(frames[f]));
+        }
+        for (uint32_t f = f_lo; f < f_hi && max_wh; f += 65535u) {
+            const uint32_t nf = std::min<uint32_t>(65535u, f_hi - f);
+            hipLaunchKernelGGL(k_winner_global, dim3(blocks_for(max_wh), nf), dim3(256), 0, s, d_draws.as<Draw>(),
+                               d_ramps.as<Ramp>(), d_frames.as<Frame>(), f, d_keys.as<uint64_t>(), d_z.as<float>(), win,
                                win_raw ? 1u : 0u);
             HIP_TRY(hipGetLastError());
         }
         if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
+        for (uint32_t f = f_lo; f < f_hi && max_cells; f += 65535u) {
+            const uint32_t nf = std::min<uint32_t>(65535u, f_hi - f);
+            hipLaunchKernelGGL(k_resolve_global, dim3(blocks_for(max_cells), nf), dim3(256), 0, s,
+                               d_frames.as<Frame>(), f, d_keys.as<uint64_t>(), d_nl.as<int32_t>(),
+                               d_cells.as<uint32_t>());
+            HIP_TRY(hipGetLastError());
+        }
+        if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));
         return SLOTH_OK;
     }
 
@@ -1091,7 +1104,7 @@ int sloth_batch_profile(sloth_batch* b, float* out_ms, const char** names, uint3
     if (b->job.path == PATH_TILED)
         nm = TiledPlan::stage_names();
     else
-        nm = {"k_clear_global", "k_raster_global", "k_resolve_global"};
+        nm = {"k_clear_global", "k_raster_global", "k_winner_global", "k_resolve_global"};
     uint32_t cap = *n_stages;
     *n_stages = (uint32_t)nm.size();
     if (!rc && se == hipSuccess) {

@@ -4,6 +4,10 @@
 //   k_raster_global   one thread per triangle: setup, classify, walk the
 //                     candidate pixels, 64-bit atomicMin of the packed key
 //                     into the frame's z-slot keys; newline rows via atomicMax
+//   k_winner_global   one thread per z slot: the winning fragment's cell and
+//                     depth, derived once from the slot's key; the depth and
+//                     winner go out, and the key's depth half is replaced by
+//                     the cell (the triangle and sub half stays)
 //   k_resolve_global  one thread per cell: last writer among the winners of
 //                     z slots c and c-1 and the row's newline
 //
@@ -123,67 +127,88 @@ __device__ __forceinline__ void winner_fragment(const Draw* __restrict__ draws, 
            ((uint32_t)c[2] << 24);
 }
 
-__global__ void __launch_bounds__(256) k_resolve_global(const Draw* __restrict__ draws, const Ramp* __restrict__ ramps,
-                                                        const Frame* __restrict__ frames, uint32_t f_lo,
-                                                        uint32_t f_hi, uint64_t cell_base, uint64_t n_cells,
-                                                        const uint64_t* __restrict__ keys,
-                                                        const int32_t* __restrict__ nl, uint32_t* __restrict__ cells,
-                                                        float* __restrict__ zbuf, int32_t* __restrict__ win,
-                                                        uint32_t win_raw)
+// Keys of slots that k_winner_global has resolved: the winning cell in the
+// high word, the key's (triangle << 1 | sub) in the low word.  Frames hold
+// fewer than 2^31 - 1 triangles, so the low word is never all ones and a
+// resolved slot never reads as KEY_EMPTY.
+//
+// Both kernels: grid (cells of the largest frame / 256, frame - f_lo).
+__global__ void __launch_bounds__(256) k_winner_global(const Draw* __restrict__ draws, const Ramp* __restrict__ ramps,
+                                                       const Frame* __restrict__ frames, uint32_t f_lo,
+                                                       uint64_t* __restrict__ keys, float* __restrict__ zbuf,
+                                                       int32_t* __restrict__ win, uint32_t win_raw)
 {
-    uint64_t t_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t_id >= n_cells) return;
-    uint64_t g = cell_base + t_id;
-    uint32_t lo = f_lo, hi = f_hi - 1;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (frames[mid].cell_begin <= g) lo = mid; else hi = mid - 1;
+    const Frame& fr = frames[f_lo + blockIdx.y];
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;   // W, H <= 65535: cell ids fit 32 bits
+    if (c >= fr.W * fr.H) return;   // the frame's z slots are its first W*H cell ids
+    uint64_t* fk = keys + fr.z_off;
+    const uint64_t k = fk[c];
+    float z = F32_MAX;
+    int32_t wt = -1;
+    if (k != KEY_EMPTY) {
+        uint32_t cell, tri, px, py;
+        winner_fragment(draws, ramps, fr, (uint32_t)c, k, cell, z, tri, px, py);
+        wt = win_raw ? (int32_t)(uint32_t)k : (int32_t)tri;   // raw: triangle << 1 | sub
+        fk[c] = ((uint64_t)cell << 32) | (uint32_t)k;
     }
-    const Frame& fr = frames[lo];
+    // streaming stores: outputs are written once
+    __builtin_nontemporal_store(z, zbuf + fr.z_off + c);
+    if (win) __builtin_nontemporal_store(wt, win + fr.z_off + c);
+}
+
+// Write time of a resolved slot's winner in the sequential loop: triangle,
+// then the pixel's row and column (plus one; 0 = no write).  (row, col) is
+// the slot's cell; the pixel is as in key_pixel.
+__device__ __forceinline__ uint64_t slot_time(uint64_t k, uint32_t row, uint32_t col, uint32_t W)
+{
+    const uint32_t lowbits = (uint32_t)k;
+    const uint32_t px = (lowbits & 1u) ? col >> 1 : (col + W) >> 1;
+    const uint32_t py = (lowbits & 1u) ? row : row - 1;
+    return (((uint64_t)(lowbits >> 1) << 32) | ((uint64_t)py << 16) | px) + 1;
+}
+
+__global__ void __launch_bounds__(256) k_resolve_global(const Frame* __restrict__ frames, uint32_t f_lo,
+                                                        const uint64_t* __restrict__ keys,
+                                                        const int32_t* __restrict__ nl, uint32_t* __restrict__ cells)
+{
+    const Frame& fr = frames[f_lo + blockIdx.y];
     const uint32_t W = fr.W, H = fr.H;
-    const uint64_t WH = (uint64_t)W * H;
-    const uint64_t c = g - fr.cell_begin;
-    if (c >= frame_cells(fr)) return;   // padding between frames
+    const uint32_t WH = W * H;   // W, H <= 65535: cell ids fit 32 bits
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= frame_cells(fr)) return;
     const uint64_t* fk = keys + fr.z_off;
+    // cell c = (row, col); its left neighbour is (row, col - 1), or the last
+    // cell of the row above when col = 0 (cells past W*H are the newline rows'
+    // tail, whose row may be H)
+    const uint32_t row = c / W, col = c - row * W;
+    const uint32_t lrow = col ? row : row - 1, lcol = col ? col - 1 : W - 1;
 
     uint64_t best_time = 0;   // 0 = no event; event times are stored +1
     uint32_t best_cell = CELL_BLANK;
 
     if (c < WH) {
-        uint64_t k = fk[c];
-        float z = F32_MAX;
-        int32_t wt = -1;
+        const uint64_t k = fk[c];
         if (k != KEY_EMPTY) {
-            uint32_t cell, tri, px, py;
-            winner_fragment(draws, ramps, fr, (uint32_t)c, k, cell, z, tri, px, py);
-            uint64_t tm = (((uint64_t)tri << 32) | ((uint64_t)py << 16) | px) + 1;
-            best_time = tm;
-            best_cell = cell;
-            wt = win_raw ? (int32_t)(uint32_t)k : (int32_t)tri;   // raw: triangle << 1 | sub
+            best_time = slot_time(k, row, col, W);
+            best_cell = (uint32_t)(k >> 32);
         }
-        // streaming stores: outputs are written once
-        __builtin_nontemporal_store(z, zbuf + fr.z_off + c);
-        if (win) __builtin_nontemporal_store(wt, win + fr.z_off + c);
     }
     if (c >= 1 && c - 1 < WH) {
-        uint64_t k = fk[c - 1];
+        const uint64_t k = fk[c - 1];
         if (k != KEY_EMPTY) {
-            uint32_t cell, tri, px, py;
-            float z;
-            winner_fragment(draws, ramps, fr, (uint32_t)(c - 1), k, cell, z, tri, px, py);
-            uint64_t tm = (((uint64_t)tri << 32) | ((uint64_t)py << 16) | px) + 1;
+            const uint64_t tm = slot_time(k, lrow, lcol, W);
             if (tm > best_time) {
                 best_time = tm;
-                best_cell = cell;
+                best_cell = (uint32_t)(k >> 32);
             }
         }
     }
-    if (fr.image && c >= 1 && (c - 1) % W == 0) {
-        uint64_t r = (c - 1) / W;
+    if (fr.image && c >= 1 && lcol == 0) {
+        const uint32_t r = lrow;
         if (r < H) {
             int32_t n = nl[fr.nl_off + r];
             if (n >= 0) {
-                uint64_t tm = (((uint64_t)(uint32_t)n << 32) | (r << 16) | 0xFFFFu) + 1;
+                uint64_t tm = (((uint64_t)(uint32_t)n << 32) | ((uint64_t)r << 16) | 0xFFFFu) + 1;
                 if (tm > best_time) {
                     best_time = tm;
                     best_cell = CELL_NEWLINE;
