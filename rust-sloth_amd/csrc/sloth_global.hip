@@ -57,9 +57,14 @@ __global__ void __launch_bounds__(256) k_raster_global(const Draw* __restrict__ 
                                                        uint64_t n_items, uint64_t* __restrict__ keys,
                                                        int32_t* __restrict__ nl)
 {
-    uint64_t t_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t_id >= n_items) return;
-    uint64_t item = item_base + t_id;
+    __shared__ int32_t s_nl[256];
+    __shared__ uint32_t s_frame[2];
+    __shared__ int s_lo, s_hi;
+    const uint64_t t_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t_id < n_items;
+    // threads past the end take the last item and do nothing with it, so
+    // that every thread reaches the block's barriers
+    const uint64_t item = item_base + (valid ? t_id : n_items - 1);
     uint32_t d = find_draw(draws, d_lo, d_hi, item);
     const Draw& dr = draws[d];
     uint32_t local = (uint32_t)(item - dr.item_begin);
@@ -70,11 +75,39 @@ __global__ void __launch_bounds__(256) k_raster_global(const Draw* __restrict__ 
     const uint32_t tri = dr.tri_begin + local;
     uint64_t* fk = keys + fr.z_off;
 
-    if (fr.image && t.y0 < t.y1d) {
-        // rasterizer.rs:89-91: every row of the domain ends with a newline cell.
-        int32_t* fnl = nl + fr.nl_off;
+    // rasterizer.rs:89-91: every row of the domain ends with a newline cell;
+    // the last triangle (in draw order) whose rows include r decides row r.
+    // A block whose items all lie in one frame (the first and the last
+    // thread's frame agree: items are in frame order) reduces its rows in
+    // LDS first, with one global atomicMax per row; others go to global
+    // memory directly.
+    const bool has_rows = valid && fr.image && t.y0 < t.y1d;
+    int32_t* fnl = nl + fr.nl_off;
+    s_nl[threadIdx.x] = -1;
+    if (threadIdx.x == 0) {
+        s_frame[0] = dr.frame;
+        s_lo = 1 << 30;
+        s_hi = -1;
+    }
+    if (threadIdx.x == blockDim.x - 1) s_frame[1] = dr.frame;
+    __syncthreads();
+    const bool one_frame = s_frame[0] == s_frame[1];
+    if (one_frame && has_rows) {
+        atomicMin(&s_lo, t.y0);
+        atomicMax(&s_hi, t.y1d - 1);
+    }
+    __syncthreads();
+    const int lo = s_lo, hi = s_hi;
+    if (one_frame && hi >= lo && hi - lo < (int)blockDim.x) {
+        if (has_rows)
+            for (int y = t.y0; y < t.y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)tri);
+        __syncthreads();
+        const int32_t v = s_nl[threadIdx.x];
+        if ((int)threadIdx.x <= hi - lo && v >= 0) atomicMax(&fnl[lo + (int)threadIdx.x], v);
+    } else if (has_rows) {
         for (int y = t.y0; y < t.y1d; ++y) atomicMax(&fnl[y], (int32_t)tri);
     }
+    if (!valid) return;
 
     uint32_t cls = setup_class(t);
     if (cls == CLS_EMPTY) return;
