@@ -73,6 +73,29 @@ def test_random_soups(sloth, W, H):
     _check(sloth, f"random soups {W}x{H}", W, H, U, draws)
 
 
+@pytest.mark.parametrize("W,H", [(301, 600), (300, 600)])
+def test_tall_frames_newline_rows(sloth, W, H):
+    """Frames taller than a block of 256 triangles can span: a UV sphere
+    (each block's triangles share a narrow band of rows, away from row 0),
+    scattered small triangles and a few that cover most of the frame (a
+    block's rows span more than 256).  The newline rows and the frame's
+    cells must match the oracle on each path."""
+    rng = np.random.default_rng(W + H)
+    from sloth_amd import meshgen
+
+    U = meshgen.screen_transform(W, H, [np.array([1.0, 1.0, 1.0], np.float32)])
+    sph = meshgen.uv_sphere(48, 40, 0.9, seed=9)
+    R0 = meshgen.rotation_euler(0.3, 1.1, 0.2)
+    draws = [(sph.xyz, sph.rgb, R0, None)]
+    # stretched 3.5x in y: the frame's rows hold about +-3.7 model units
+    S = np.diag([1.0, 3.5, 1.0, 1.0]).astype(np.float32).reshape(16)
+    xyz, rgb = random_soup(rng, 700, W, H, U, spread=1.0, size=0.03)
+    draws.append((xyz, rgb, S, None))
+    big = rng.uniform(-1.0, 1.0, size=(40, 3, 3)).astype(np.float32)
+    draws.append((big, rng.integers(0, 256, size=(40, 3), dtype=np.uint8), S, None))
+    _check(sloth, f"tall frame {W}x{H}", W, H, U, draws)
+
+
 def test_custom_shaders_per_draw(sloth):
     from sloth_amd import meshgen, Shader
 
