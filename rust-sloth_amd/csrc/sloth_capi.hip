@@ -328,8 +328,6 @@ struct Job {
         uint64_t z_lo = a.z_off, z_hi = b.z_off + (uint64_t)b.W * b.H;
         uint64_t r_lo = a.nl_off, r_hi = b.nl_off + b.H;
         uint32_t d_lo = a.draw_begin, d_hi = b.draw_end;
-        uint64_t it_lo = d_lo < draws.size() ? draws[d_lo].item_begin : n_items;
-        uint64_t it_hi = d_hi < draws.size() ? draws[d_hi].item_begin : n_items;
         if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
         {
             uint64_t n = std::max(z_hi - z_lo, r_hi - r_lo);
@@ -339,10 +337,13 @@ struct Job {
             HIP_TRY(hipGetLastError());
         }
         if (ev) HIP_TRY(hipEventRecord((*ev)[1], s));
-        if (it_hi > it_lo) {
-            hipLaunchKernelGGL(k_raster_global, dim3(blocks_for(it_hi - it_lo)), dim3(256), 0, s, d_draws.as<Draw>(),
-                               d_lo, d_hi, d_frames.as<Frame>(), it_lo, it_hi - it_lo, d_keys.as<uint64_t>(),
-                               d_nl.as<int32_t>());
+        // grid (blocks of the largest draw, draws), at most 65535 draws per launch
+        uint32_t max_n = 0;
+        for (uint32_t d = d_lo; d < d_hi; ++d) max_n = std::max(max_n, draws[d].n);
+        for (uint32_t d = d_lo; d < d_hi && max_n; d += 65535u) {
+            const uint32_t nd = std::min<uint32_t>(65535u, d_hi - d);
+            hipLaunchKernelGGL(k_raster_global, dim3(blocks_for(max_n), nd), dim3(256), 0, s, d_draws.as<Draw>(), d,
+                               d_frames.as<Frame>(), d_keys.as<uint64_t>(), d_nl.as<int32_t>());
             HIP_TRY(hipGetLastError());
         }
         if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));

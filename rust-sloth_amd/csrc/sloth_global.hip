@@ -27,18 +27,6 @@ __global__ void k_clear_global(uint64_t* __restrict__ keys, uint64_t n_keys, int
         nl[i] = -1;
 }
 
-// Index of the draw that owns launch-wide triangle `item`.
-__device__ __forceinline__ uint32_t find_draw(const Draw* __restrict__ draws, uint32_t d_lo, uint32_t d_hi,
-                                              uint64_t item)
-{
-    uint32_t lo = d_lo, hi = d_hi - 1;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (draws[mid].item_begin <= item) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ void plot(uint64_t* __restrict__ keys, const Tri& t, uint32_t W, uint32_t tri, int x,
                                      int y)
 {
@@ -52,22 +40,21 @@ __device__ __forceinline__ void plot(uint64_t* __restrict__ keys, const Tri& t, 
     atomicMin((unsigned long long*)&keys[id], (unsigned long long)make_key(z, tri, sub));
 }
 
-__global__ void __launch_bounds__(256) k_raster_global(const Draw* __restrict__ draws, uint32_t d_lo, uint32_t d_hi,
-                                                       const Frame* __restrict__ frames, uint64_t item_base,
-                                                       uint64_t n_items, uint64_t* __restrict__ keys,
+// Grid: x = blocks of one draw's triangles, y = draw index - d_lo, so the
+// draw, its frame and its matrix are uniform per block (scalar loads).
+__global__ void __launch_bounds__(256) k_raster_global(const Draw* __restrict__ draws, uint32_t d_lo,
+                                                       const Frame* __restrict__ frames, uint64_t* __restrict__ keys,
                                                        int32_t* __restrict__ nl)
 {
     __shared__ int32_t s_nl[256];
-    __shared__ uint32_t s_frame[2];
     __shared__ int s_lo, s_hi;
-    const uint64_t t_id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = t_id < n_items;
-    // threads past the end take the last item and do nothing with it, so
-    // that every thread reaches the block's barriers
-    const uint64_t item = item_base + (valid ? t_id : n_items - 1);
-    uint32_t d = find_draw(draws, d_lo, d_hi, item);
-    const Draw& dr = draws[d];
-    uint32_t local = (uint32_t)(item - dr.item_begin);
+    const Draw& dr = draws[d_lo + blockIdx.y];
+    const uint32_t base = blockIdx.x * blockDim.x;
+    if (base >= dr.n) return;   // the whole block: past this draw's triangles
+    const bool valid = base + threadIdx.x < dr.n;
+    // threads past the end take the draw's last triangle and do nothing with
+    // it, so that every thread reaches the block's barriers
+    const uint32_t local = valid ? base + threadIdx.x : dr.n - 1;
     const Frame& fr = frames[dr.frame];
     const uint32_t W = fr.W, H = fr.H;
     Tri t;
@@ -77,28 +64,24 @@ __global__ void __launch_bounds__(256) k_raster_global(const Draw* __restrict__ 
 
     // rasterizer.rs:89-91: every row of the domain ends with a newline cell;
     // the last triangle (in draw order) whose rows include r decides row r.
-    // A block whose items all lie in one frame (the first and the last
-    // thread's frame agree: items are in frame order) reduces its rows in
-    // LDS first, with one global atomicMax per row; others go to global
-    // memory directly.
+    // The block reduces its rows in LDS first, with one global atomicMax per
+    // row, when they span fewer than 256; otherwise each triangle's rows go
+    // to global memory directly.
     const bool has_rows = valid && fr.image && t.y0 < t.y1d;
     int32_t* fnl = nl + fr.nl_off;
     s_nl[threadIdx.x] = -1;
     if (threadIdx.x == 0) {
-        s_frame[0] = dr.frame;
         s_lo = 1 << 30;
         s_hi = -1;
     }
-    if (threadIdx.x == blockDim.x - 1) s_frame[1] = dr.frame;
     __syncthreads();
-    const bool one_frame = s_frame[0] == s_frame[1];
-    if (one_frame && has_rows) {
+    if (has_rows) {
         atomicMin(&s_lo, t.y0);
         atomicMax(&s_hi, t.y1d - 1);
     }
     __syncthreads();
     const int lo = s_lo, hi = s_hi;
-    if (one_frame && hi >= lo && hi - lo < (int)blockDim.x) {
+    if (hi >= lo && hi - lo < (int)blockDim.x) {
         if (has_rows)
             for (int y = t.y0; y < t.y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)tri);
         __syncthreads();
