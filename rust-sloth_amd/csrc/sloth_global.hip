@@ -130,17 +130,24 @@ __device__ __forceinline__ void winner_fragment(const Draw* __restrict__ draws, 
     uint32_t lowbits = (uint32_t)key;
     tri = lowbits >> 1;
     key_pixel(id, lowbits & 1u, fr.W, px, py);
-    const Draw& dr = draw_of(draws, fr, tri);
-    uint32_t local = tri - dr.tri_begin;
-    Tri t;
-    setup_tri(dr.M, dr.xyz + (size_t)local * 9, fr.W, fr.H, t);
-    float w0, w1, w2;
-    edges(t, (float)px, (float)py, w0, w1, w2);
-    z = depth(t, w1, w2);
-    float s = shade_value(unit_normal_z(t), t.a, w0, w1, w2);
-    const auto* c = global_ptr(dr.rgb + (size_t)local * 3);
-    cell = (uint32_t)ramp_glyph(ramps[dr.ramp], s) | ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) |
-           ((uint32_t)c[2] << 24);
+    auto derive = [&](const Draw& dr) {
+        uint32_t local = tri - dr.tri_begin;
+        Tri t;
+        setup_tri(dr.M, dr.xyz + (size_t)local * 9, fr.W, fr.H, t);
+        float w0, w1, w2;
+        edges(t, (float)px, (float)py, w0, w1, w2);
+        z = depth(t, w1, w2);
+        float s = shade_value(unit_normal_z(t), t.a, w0, w1, w2);
+        const auto* c = global_ptr(dr.rgb + (size_t)local * 3);
+        cell = (uint32_t)ramp_glyph(ramps[dr.ramp], s) | ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) |
+               ((uint32_t)c[2] << 24);
+    };
+    // a frame of one draw (the usual case) reads it at a uniform address
+    // (scalar loads of the matrix and the shader table): two copies
+    if (fr.draw_end - fr.draw_begin == 1)
+        derive(draws[fr.draw_begin]);
+    else
+        derive(draw_of(draws, fr, tri));
 }
 
 // Keys of slots that k_winner_global has resolved: the winning cell in the
