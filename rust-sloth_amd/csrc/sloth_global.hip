@@ -3,7 +3,8 @@
 //   k_clear_global    z-slot keys <- empty, newline table <- -1
 //   k_raster_global   one thread per triangle: setup, classify, walk the
 //                     candidate pixels, 64-bit atomicMin of the packed key
-//                     into the frame's z-slot keys; newline rows via atomicMax
+//                     into the frame's z-slot keys; newline rows by atomicMax,
+//                     reduced per block in LDS first
 //   k_winner_global   one thread per z slot: the winning fragment's cell and
 //                     depth, derived once from the slot's key; the depth and
 //                     winner go out, and the key's depth half is replaced by
