@@ -303,15 +303,15 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     }
     // rasterizer.rs:89-91: a newline cell after every row of the domain; the
     // last triangle (in draw order) whose rows include r decides row r.
+    // The block reduces its rows in LDS, one global atomicMax per row: here
+    // when they span fewer than BIN_BLOCK rows, else in windows of BIN_BLOCK
+    // rows at the end (tall triangles put every row of a tall frame under
+    // the atomics of each triangle that covers it).
     const int lo = s_lo, hi = s_hi;
     const bool nl_lds = hi >= lo && hi - lo < BIN_BLOCK;
     int32_t* fnl = nl + fr.nl_off;
-    if (has_rows) {
-        if (nl_lds)
-            for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)tri);
-        else
-            for (int y = y0; y < y1d; ++y) atomicMax(&fnl[y], (int32_t)tri);
-    }
+    if (has_rows && nl_lds)
+        for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)tri);
     __syncthreads();
 
     {
@@ -349,6 +349,19 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
             const uint32_t pos = atomicAdd(&cursor[gt], 1u);
             put_entry<COUNT>(gt, pos, entry, off, pool, overflow, frame);
             iter_next(r, it);
+        }
+    }
+    if (hi >= lo && !nl_lds) {   // uniform per block
+        for (int ws = lo; ws <= hi; ws += BIN_BLOCK) {
+            __syncthreads();   // the previous window's reads are done
+            s_nl[threadIdx.x] = -1;
+            __syncthreads();
+            if (has_rows)
+                for (int y = max(y0, ws); y < min(y1d, ws + BIN_BLOCK); ++y)
+                    atomicMax(&s_nl[y - ws], (int32_t)tri);
+            __syncthreads();
+            const int32_t v = s_nl[threadIdx.x];
+            if (ws + (int)threadIdx.x <= hi && v >= 0) atomicMax(&fnl[ws + (int)threadIdx.x], v);
         }
     }
 }
