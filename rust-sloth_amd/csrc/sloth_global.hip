@@ -65,9 +65,8 @@ __global__ void __launch_bounds__(256) k_raster_global(const Draw* __restrict__ 
 
     // rasterizer.rs:89-91: every row of the domain ends with a newline cell;
     // the last triangle (in draw order) whose rows include r decides row r.
-    // The block reduces its rows in LDS first, with one global atomicMax per
-    // row, when they span fewer than 256; otherwise each triangle's rows go
-    // to global memory directly.
+    // The block reduces its rows in LDS, 256 rows at a time, with one global
+    // atomicMax per row and window.
     const bool has_rows = valid && fr.image && t.y0 < t.y1d;
     int32_t* fnl = nl + fr.nl_off;
     s_nl[threadIdx.x] = -1;
@@ -81,15 +80,19 @@ __global__ void __launch_bounds__(256) k_raster_global(const Draw* __restrict__ 
         atomicMax(&s_hi, t.y1d - 1);
     }
     __syncthreads();
-    const int lo = s_lo, hi = s_hi;
-    if (hi >= lo && hi - lo < (int)blockDim.x) {
+    const int lo = s_lo, hi = s_hi;   // uniform per block
+    for (int ws = lo; ws <= hi; ws += (int)blockDim.x) {
+        if (ws > lo) {
+            __syncthreads();   // the previous window's reads are done
+            s_nl[threadIdx.x] = -1;
+            __syncthreads();
+        }
         if (has_rows)
-            for (int y = t.y0; y < t.y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)tri);
+            for (int y = max(t.y0, ws); y < min(t.y1d, ws + (int)blockDim.x); ++y)
+                atomicMax(&s_nl[y - ws], (int32_t)tri);
         __syncthreads();
         const int32_t v = s_nl[threadIdx.x];
-        if ((int)threadIdx.x <= hi - lo && v >= 0) atomicMax(&fnl[lo + (int)threadIdx.x], v);
-    } else if (has_rows) {
-        for (int y = t.y0; y < t.y1d; ++y) atomicMax(&fnl[y], (int32_t)tri);
+        if (ws + (int)threadIdx.x <= hi && v >= 0) atomicMax(&fnl[ws + (int)threadIdx.x], v);
     }
     if (!valid) return;
 
