@@ -238,26 +238,16 @@ def leg_shading(sloth_amd, meshgen):
     seeded views at 1024x1024, L2 loss against a target rendered with colours
     + N(0, 0.1^2) and the light rotated 10 degrees; one step = forward +
     backward over all 32 views with visibility fixed."""
-    import math
-
     log("bench: shading fwd+bwd leg")
-    m = meshgen.uv_sphere(224, 224, 1.0, seed=5)
-    rng = np.random.default_rng(5)
-    W = H = 1024
-    V = 32
-    U = meshgen.screen_transform(W, H, [m.bbox_max()])
+    wl = meshgen.workload_shading_views()
+    m, W, H, U, Rs = wl["mesh"], wl["W"], wl["H"], wl["U"], wl["Rs"]
+    V = len(Rs)
     mesh = sloth_amd.SimpleMesh.from_mesh(m)
-    Rs = [meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3)) for _ in range(V)]
     b = sloth_amd.Batch([sloth_amd.FrameSpec(W, H, U, True) for _ in range(V)],
                         [sloth_amd.DrawSpec(f, mesh, Rs[f]) for f in range(V)])
     sh = sloth_amd.Shading(b, m.faces, m.verts.shape[0])
-    vc = rng.uniform(0, 1, (m.verts.shape[0], 3)).astype(np.float32)
-    vn = (m.verts / np.linalg.norm(m.verts, axis=1, keepdims=True)).astype(np.float32)
-    L = np.array([0.577, 0.577, -0.577], np.float32)
-    a = math.radians(10.0)
-    Ry = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
-    _, target = sh.forward((vc + rng.normal(0, 0.1, vc.shape)).astype(np.float32), vn,
-                           (Ry @ L.astype(np.float64)).astype(np.float32), want_out=True)
+    vc, vn, L = wl["vcolor"], wl["vnormal"], wl["light"]
+    _, target = sh.forward(wl["vcolor_target"], vn, wl["light_target"], want_out=True)
     sh.set_target(target)
     loss, _ = sh.forward(vc, vn, L)
     ms = sh.time(10, 2) / 10

@@ -147,7 +147,9 @@ int sloth_draw_mesh(sloth_context *ctx, const sloth_mesh *mesh, const float *tra
 
 /* Rasterizes every draw recorded since the last clear into the context's
  * device buffers, on `stream` (a hipStream_t; NULL = the context's own
- * stream).  Returns without waiting for the device. */
+ * stream).  Returns without waiting for the device.  On a caller's stream
+ * the launch waits for the context's pending uploads, and the context's
+ * stream waits for the launch, so sloth_context_read waits for it too. */
 int sloth_context_render(sloth_context *ctx, void *stream);
 
 /* Renders if needed, then copies the frame buffer (W*H (+H) cells), z buffer
@@ -202,12 +204,21 @@ typedef struct sloth_draw_desc {
 int sloth_batch_create(const sloth_frame_desc *frames, uint32_t n_frames, const sloth_draw_desc *draws,
                        uint32_t n_draws, const sloth_ramp *shader, sloth_batch **out);
 
-/* Replaces the per-draw transforms (float[n_draws][16]) without rebuilding. */
+/* Replaces the per-draw transforms without rebuilding: transforms16 holds
+ * exactly n_draws column-major matrices, float[n_draws][16], in the order of
+ * the draw array given to sloth_batch_create.  Waits for the batch's stream.
+ * The frames on the device keep the old transforms' images until the next
+ * sloth_batch_run / _run_per_frame; reading before that returns SLOTH_ESTATE.
+ * If the new transforms need longer tile lists than the ones sized so far,
+ * the next synchronising call (sloth_batch_sync, _read_frame, _encode_frame)
+ * re-sizes them, renders again and re-captures the graph at the next run. */
 int sloth_batch_set_transforms(sloth_batch *b, const float *transforms16);
 
 /* Renders all frames on `stream` (NULL = the batch's own stream).  The first
- * call on a stream captures the launch sequence into a HIP graph; later calls
- * replay it.  Returns without waiting. */
+ * call captures the launch sequence into a HIP graph; later calls replay it.
+ * Returns without waiting.  On a caller's stream the launch waits for the
+ * batch's own pending work, and the batch's stream waits for the launch, so
+ * the synchronising calls below also wait for it. */
 int sloth_batch_run(sloth_batch *b, void *stream);
 
 /* Renders every frame with one launch sequence per frame and a device

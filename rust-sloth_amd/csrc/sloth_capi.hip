@@ -431,6 +431,28 @@ static hipStream_t make_stream()
     return s;
 }
 
+static hipEvent_t make_event()
+{
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+
+// Makes `to` wait for everything enqueued on `from` so far.  A handle's
+// uploads, overflow checks and read-backs run on its own stream; a render
+// launched on a caller's stream is joined to it in both directions, so those
+// never overlap the render.
+static int join_streams(hipStream_t from, hipStream_t to, hipEvent_t ev)
+{
+    if (from == to) return SLOTH_OK;
+    HIP_TRY(hipEventRecord(ev, from));
+    HIP_TRY(hipStreamWaitEvent(to, ev, 0));
+    return SLOTH_OK;
+}
+
 // Device scratch of the flush encoders.
 struct Encoder {
     DevBuf d_off, d_block, d_total, d_out;
@@ -483,12 +505,14 @@ struct sloth_context {
     Job job;
     float U[16];
     hipStream_t stream = nullptr;
+    hipEvent_t join_ev = nullptr;   // joins a caller's stream to `stream`
     bool rendered = false;      // device buffers match the draw list
     bool win_valid = false;     // d_win matches the draw list
     bool checked = false;       // the last render was checked for tile-list overflow
     Encoder enc;
     ~sloth_context()
     {
+        if (join_ev) (void)hipEventDestroy(join_ev);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -502,6 +526,7 @@ struct sloth_batch {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;       // second branch of the captured pipeline
     std::vector<hipEvent_t> join;        // its fork / join events
+    hipEvent_t join_ev = nullptr;        // joins a caller's stream to `stream`
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t exec_generation = 0;   // tile plan generation the graph was captured with
@@ -515,6 +540,7 @@ struct sloth_batch {
         if (graph) (void)hipGraphDestroy(graph);
         for (hipEvent_t e : join)
             if (e) (void)hipEventDestroy(e);
+        if (join_ev) (void)hipEventDestroy(join_ev);
         if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -752,9 +778,10 @@ int sloth_context_create(uint32_t width, uint32_t height, int image, sloth_conte
     memcpy(c->U, IDENTITY, sizeof c->U);
     c->job.fin.push_back(FrameIn{width, height, image ? 1 : 0});
     c->stream = make_stream();
-    if (!c->stream) {
+    c->join_ev = make_event();
+    if (!c->stream || !c->join_ev) {
         delete c;
-        return fail(SLOTH_EHIP, "hipStreamCreate failed");
+        return fail(SLOTH_EHIP, "hipStreamCreate / hipEventCreate failed");
     }
     *out = c;
     return SLOTH_OK;
@@ -824,13 +851,17 @@ static int context_render(sloth_context* ctx, hipStream_t s, bool want_win)
         if ((rc = ctx->job.upload(ctx->stream))) return rc;
         if ((rc = ctx->job.prepare(ctx->stream))) return rc;
         if (want_win && (rc = ctx->job.ensure_win())) return rc;
+        if ((rc = join_streams(ctx->stream, s, ctx->join_ev))) return rc;
         if ((rc = ctx->job.launch(0, 1, s, want_win ? ctx->job.d_win.as<int32_t>() : nullptr))) return rc;
+        if ((rc = join_streams(s, ctx->stream, ctx->join_ev))) return rc;
         ctx->rendered = true;
         ctx->checked = false;
         ctx->win_valid = want_win;
     } else if (want_win && !ctx->win_valid) {
         if ((rc = ctx->job.ensure_win())) return rc;
+        if ((rc = join_streams(ctx->stream, s, ctx->join_ev))) return rc;
         if ((rc = ctx->job.launch(0, 1, s, ctx->job.d_win.as<int32_t>()))) return rc;
+        if ((rc = join_streams(s, ctx->stream, ctx->join_ev))) return rc;
         ctx->checked = false;
         ctx->win_valid = true;
     }
@@ -915,9 +946,10 @@ int sloth_batch_create(const sloth_frame_desc* frames, uint32_t n_frames, const 
     if (!b) return fail(SLOTH_ENOMEM, "host allocation failed");
     b->stream = make_stream();
     b->stream2 = make_stream();
-    if (!b->stream || !b->stream2) {
+    b->join_ev = make_event();
+    if (!b->stream || !b->stream2 || !b->join_ev) {
         delete b;
-        return fail(SLOTH_EHIP, "hipStreamCreate failed");
+        return fail(SLOTH_EHIP, "hipStreamCreate / hipEventCreate failed");
     }
     b->join.assign(MAX_GROUPS + 1, nullptr);
     for (hipEvent_t& e : b->join)
@@ -961,6 +993,10 @@ int sloth_batch_set_transforms(sloth_batch* b, const float* transforms16)
     // the draw table is grouped by frame; recompute it in the same order
     int rc;
     if ((rc = b->job.layout()) || (rc = b->job.upload(b->stream))) return rc;
+    // the frames on the device show the old transforms: a read must follow a
+    // run, whose overflow check covers the new ones
+    b->ran = false;
+    b->checked = false;
     std::fill(b->win_valid.begin(), b->win_valid.end(), false);
     return SLOTH_OK;
 }
@@ -1007,7 +1043,9 @@ int sloth_batch_run(sloth_batch* b, void* stream)
         HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->exec_generation = b->job.tiled.generation;
     }
+    if ((rc = join_streams(b->stream, s, b->join_ev))) return rc;
     HIP_TRY(hipGraphLaunch(b->exec, s));
+    if ((rc = join_streams(s, b->stream, b->join_ev))) return rc;
     b->ran = true;
     b->checked = false;
     std::fill(b->win_valid.begin(), b->win_valid.end(), false);
@@ -1021,6 +1059,7 @@ int sloth_batch_run_per_frame(sloth_batch* b, void* stream)
     if (rc) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : b->stream;
     if ((rc = b->job.prepare(b->stream))) return rc;
+    if ((rc = join_streams(b->stream, s, b->join_ev))) return rc;
     for (uint32_t f = 0; f < (uint32_t)b->job.frames.size(); ++f) {
         if ((rc = b->job.launch(f, f + 1, s, nullptr))) return rc;
         HIP_TRY(hipStreamSynchronize(s));
@@ -1057,6 +1096,11 @@ int sloth_batch_read_frame(sloth_batch* b, uint32_t frame, uint32_t* cells, floa
     if (winners && !b->win_valid[frame]) {
         if ((rc = b->job.ensure_win())) return rc;
         if ((rc = b->job.launch(frame, frame + 1, s, b->job.d_win.as<int32_t>()))) return rc;
+        // the lists were sized for these transforms by the checked run, but a
+        // re-plan here must still re-render the frame and drop the graph
+        bool replanned = false;
+        if ((rc = b->job.finish(frame, frame + 1, s, b->job.d_win.as<int32_t>(), &replanned))) return rc;
+        if (replanned) drop_graph(b);
         b->win_valid[frame] = true;
     }
     if (cells)
@@ -1406,6 +1450,7 @@ int sloth_shading_visibility(sloth_shading* sh)
     Job& j = sh->b->job;
     std::vector<float> G(j.draws.size() * 9);
     for (size_t d = 0; d < j.draws.size(); ++d) normal_map(j.draws[d].M, &G[d * 9]);
+    HIP_TRY(hipStreamSynchronize(sh->b->stream));   // no shading pass still reads the old maps
     if (!G.empty()) HIP_TRY(hipMemcpy(sh->d_G.p, G.data(), G.size() * 4, hipMemcpyHostToDevice));
     return shading_visibility(sh);
 }

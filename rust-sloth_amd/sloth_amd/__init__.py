@@ -476,7 +476,12 @@ class Batch:
         self._keep = [d.mesh for d in draws]
 
     def set_transforms(self, transforms) -> None:
-        t = np.ascontiguousarray(transforms, np.float32).reshape(-1, 16)
+        """One column-major 4x4 per draw, in the order the draws were given;
+        the next run renders with them."""
+        t = np.ascontiguousarray(transforms, np.float32)
+        if t.size != 16 * len(self.draws):
+            raise ValueError(f"set_transforms needs {len(self.draws)} matrices of 16 floats, got {t.size} floats")
+        t = t.reshape(-1, 16)
         _check(load_library().sloth_batch_set_transforms(self._h, _fptr(t)))
 
     def run(self, stream: int | None = None) -> None:
@@ -556,6 +561,10 @@ class Shading:
         n = C.c_uint64()
         _check(L.sloth_shading_slots(h, C.byref(n)))
         self.n_slots = int(n.value)
+
+    def visibility(self) -> None:
+        """Re-renders the visibility after Batch.set_transforms."""
+        _check(load_library().sloth_shading_visibility(self._h))
 
     def set_target(self, target) -> None:
         t = None if target is None else np.ascontiguousarray(target, np.float32).reshape(self.n_slots, 3)

@@ -12,6 +12,8 @@ so a mesh is a pure function of its arguments.
 """
 from __future__ import annotations
 
+import ctypes
+import ctypes.util
 import math
 from dataclasses import dataclass
 
@@ -207,14 +209,34 @@ def _f32(x) -> np.float32:
     return np.float32(x)
 
 
+_libm = None
+
+
+def _sin_cos_f32(x) -> tuple:
+    """Rust's f32::sin_cos on Linux: the C library's binary32 sinf and cosf
+    (not float64 sin/cos rounded to f32, which differs in the last bit for
+    some angles)."""
+    global _libm
+    if _libm is None:
+        lib = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+        for name in ("sinf", "cosf"):
+            fn = getattr(lib, name)
+            fn.restype = ctypes.c_float
+            fn.argtypes = [ctypes.c_float]
+        _libm = lib
+    a = float(np.float32(x))
+    return np.float32(_libm.sinf(a)), np.float32(_libm.cosf(a))
+
+
 def rotation_euler(roll: float, pitch: float, yaw: float) -> np.ndarray:
     """Rotation3::from_euler_angles(roll, pitch, yaw).to_homogeneous()
-    (main.rs:76-77) in binary32, as a column-major float32[16].  sin/cos are
-    rounded to f32 from float64 values."""
+    (main.rs:76-77) in binary32, as a column-major float32[16], with the
+    angles' sines and cosines from the C library's sinf/cosf as Rust's
+    f32::sin_cos gives them."""
+    sr, cr = _sin_cos_f32(roll)
+    sp, cp = _sin_cos_f32(pitch)
+    sy, cy = _sin_cos_f32(yaw)
     f = _f32
-    sr, cr = f(math.sin(f(roll))), f(math.cos(f(roll)))
-    sp, cp = f(math.sin(f(pitch))), f(math.cos(f(pitch)))
-    sy, cy = f(math.sin(f(yaw))), f(math.cos(f(yaw)))
     with np.errstate(all="ignore"):
         m00 = cy * cp
         m01 = cy * sp * sr - sy * cr
@@ -318,3 +340,25 @@ def workload_uneven_meshes(n_meshes: int = 64, seed: int = 3, lo: float = 1e3, h
         U = screen_transform(W, H, [m.bbox_max()])
         items.append((m, W, H, U, R))
     return items
+
+
+def workload_shading_views(n_views: int = 32, size: int = 1024, n_lon: int = 224, n_lat: int = 224,
+                           seed: int = 5):
+    """The differentiable-shading workload: a UV sphere (224 x 224 = 99,904
+    triangles), n_views seeded rotations at size x size image mode, per-vertex
+    colours uniform in [0, 1), unit radial normals, light (0.577, 0.577,
+    -0.577); the target's parameters are the colours + N(0, 0.1^2) and the
+    light rotated 10 degrees about y.  Returns a dict of those arrays."""
+    m = uv_sphere(n_lon, n_lat, 1.0, seed=seed)
+    rng = np.random.default_rng(seed)
+    U = screen_transform(size, size, [m.bbox_max()])
+    Rs = [rotation_euler(*rng.uniform(0, 2 * math.pi, 3)) for _ in range(n_views)]
+    vc = rng.uniform(0, 1, (m.verts.shape[0], 3)).astype(np.float32)
+    vn = (m.verts / np.linalg.norm(m.verts, axis=1, keepdims=True)).astype(np.float32)
+    light = np.array([0.577, 0.577, -0.577], np.float32)
+    a = math.radians(10.0)
+    Ry = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+    vc_target = (vc + rng.normal(0, 0.1, vc.shape)).astype(np.float32)
+    light_target = (Ry @ light.astype(np.float64)).astype(np.float32)
+    return {"mesh": m, "W": size, "H": size, "U": U, "Rs": Rs, "vcolor": vc, "vnormal": vn, "light": light,
+            "vcolor_target": vc_target, "light_target": light_target}

@@ -43,3 +43,35 @@ def test_turntable_schedule():
         assert p[k] == acc
         acc = np.float32(acc + step)
     assert meshgen.turntable_pitches(1, 0.3)[0] == np.float32(np.float32(0.3) + pi)
+
+
+def test_rotation_matches_libm_sinf_cosf():
+    """The workloads' model matrices are Rotation3::from_euler_angles in
+    binary32 with the C library's sinf/cosf (main.rs:76-77, Rust's
+    f32::sin_cos): bit for bit the oracle's, over every pitch of the 256-frame
+    turntable and 2000 random Euler triples.  float64 sin/cos rounded to f32
+    differ from sinf/cosf in the last bit on 7 of those pitches, so this test
+    fails for that construction."""
+    import oracle
+
+    for p in meshgen.turntable_pitches(256):
+        a = meshgen.rotation_euler(0.0, float(p), 0.0)
+        b = oracle.rotation_euler(0.0, float(p), 0.0)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), p
+    rng = np.random.default_rng(11)
+    for roll, pitch, yaw in rng.uniform(-7.0, 7.0, size=(2000, 3)).astype(np.float32):
+        a = meshgen.rotation_euler(float(roll), float(pitch), float(yaw))
+        b = oracle.rotation_euler(float(roll), float(pitch), float(yaw))
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (roll, pitch, yaw)
+
+
+def test_rotation_f64_rounding_differs():
+    """Guard for the test above: the float64-rounded construction is a
+    different function on the turntable (so the pin has teeth)."""
+    import math
+
+    diff = 0
+    for p in meshgen.turntable_pitches(256):
+        s, c = meshgen._sin_cos_f32(p)
+        diff += int(s != np.float32(math.sin(float(p))) or c != np.float32(math.cos(float(p))))
+    assert diff > 0
