@@ -72,8 +72,20 @@ def lib():
         L.oracle_encode.restype = C.c_size_t
         L.oracle_shader_exhaustive_mismatch.argtypes = [C.c_uint32, C.c_uint32]
         L.oracle_shader_exhaustive_mismatch.restype = C.c_uint32
+        L.oracle_set_arith_variant.argtypes = [C.c_int]
         _lib = L
     return _lib
+
+
+NORM_SEQUENTIAL = 1    # norm_squared as ((x*x + y*y) + z*z) + w*w
+MATVEC_PAIRWISE = 2    # Matrix4 * Vector4 rows as (m0 x + m2 z) + (m1 y + m3 w)
+
+
+def set_arith_variant(v: int) -> None:
+    """Alternative operation orders for the oracle's calls on the calling
+    thread (0 = the restatement of nalgebra 0.22's order).  For bounding what
+    the unpinned order could change; never used to check the GPU."""
+    lib().oracle_set_arith_variant(int(v))
 
 
 def _p(a: np.ndarray, t):

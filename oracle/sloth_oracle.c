@@ -54,11 +54,29 @@ void oracle_mat4_mul(const float *A, const float *B, float *out)
     memcpy(out, r, sizeof r);
 }
 
+/* Alternative operation orders, for bounding what the unpinned nalgebra
+ * order could change (tests only; per calling thread, 0 = the restatement
+ * above and below):
+ *   ORACLE_NORM_SEQUENTIAL  norm_squared as ((x*x + y*y) + z*z) + w*w
+ *                           instead of the 4-wide dot's (x*x + z*z) + (y*y + w*w)
+ *   ORACLE_MATVEC_PAIRWISE  Matrix4 * Vector4 rows as the 4-wide dot's
+ *                           (M(i,0) x + M(i,2) z) + (M(i,1) y + M(i,3) w)
+ *                           instead of gemv's ((M(i,0) x + M(i,1) y) + M(i,2) z) + M(i,3) w */
+static __thread int g_variant = 0;
+
+void oracle_set_arith_variant(int v) { g_variant = v; }
+
 /* Matrix4 * Vector4 (geometry.rs:44-46), same gemv order. */
 static inline void mat4_vec(const float *M, const float *v, float *out)
 {
     float r[4];
     for (int i = 0; i < 4; ++i) {
+        if (g_variant & ORACLE_MATVEC_PAIRWISE) {
+            float a = M[0 * 4 + i] * v[0], b = M[1 * 4 + i] * v[1];
+            float c = M[2 * 4 + i] * v[2], d = M[3 * 4 + i] * v[3];
+            r[i] = (a + c) + (b + d);
+            continue;
+        }
         float acc = M[0 * 4 + i] * v[0];
         acc = M[1 * 4 + i] * v[1] + acc;
         acc = M[2 * 4 + i] * v[2] + acc;
@@ -131,9 +149,14 @@ void oracle_tri_normal(const float *tri, float *out)
     float z = (e1[0] * e2[1]) - (e1[1] * e2[0]);
     float w = 0.0f;
     float a = x * x, b = y * y, c = z * z, d = w * w;
-    a += c;
-    b += d;
-    float n2 = 0.0f + (a + b);
+    float n2;
+    if (g_variant & ORACLE_NORM_SEQUENTIAL) {
+        n2 = ((a + b) + c) + d;
+    } else {
+        a += c;
+        b += d;
+        n2 = 0.0f + (a + b);
+    }
     float n = sqrtf(n2);
     out[0] = x / n;
     out[1] = y / n;

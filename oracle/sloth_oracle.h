@@ -18,6 +18,9 @@ typedef struct {
     uint8_t pad[3];
 } oracle_ramp;
 
+#define ORACLE_NORM_SEQUENTIAL 1
+#define ORACLE_MATVEC_PAIRWISE 2
+void oracle_set_arith_variant(int v);
 void oracle_mat4_mul(const float *A, const float *B, float *out);
 void oracle_rotation_euler(float roll, float pitch, float yaw, float *out);
 void oracle_tri_mul(const float *M, float *tri);

@@ -109,32 +109,52 @@ def test_shading_visibility_after_new_transforms(sloth):
         assert np.abs(out[f * nz:(f + 1) * nz] - o).max() < 1e-5
 
 
+def _hip():
+    """The HIP runtime the library runs on (same soname, so the one already
+    loaded with it), for a caller-owned stream.  Torch is not used: it
+    bundles a second HIP runtime whose stream handles this one cannot read."""
+    import ctypes as C
+
+    sloth_lib = __import__("sloth_amd").load_library()   # noqa: F841  (loads the runtime first)
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    return hip
+
+
 def test_render_on_caller_stream(sloth):
     """Batch and context renders launched on a caller's stream, read back
     with no synchronisation of that stream by the caller: the handle's own
     stream waits for the launch."""
-    import torch
+    import ctypes as C
+
     from sloth_amd import meshgen
 
-    m = meshgen.displaced_icosphere(4, seed=5)
-    W, H, F = 640, 240, 8
-    U = meshgen.screen_transform(W, H, [m.bbox_max()])
-    mesh = sloth.SimpleMesh.from_mesh(m)
-    Rs = [meshgen.rotation_euler(0.0, 0.7 * f, 0.2) for f in range(F)]
-    frames = [sloth.FrameSpec(W, H, U, True) for _ in range(F)]
-    b = sloth.Batch(frames, [sloth.DrawSpec(f, mesh, Rs[f]) for f in range(F)])
-    s = torch.cuda.Stream()
-    say("case batch on a caller's stream")
-    for _ in range(3):
-        b.run(s.cuda_stream)
-    _check_frames(b, m, frames, Rs, W, winners=False)
+    hip = _hip()
+    s = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(s), 1) == 0   # hipStreamNonBlocking
+    try:
+        m = meshgen.displaced_icosphere(4, seed=5)
+        W, H, F = 640, 240, 8
+        U = meshgen.screen_transform(W, H, [m.bbox_max()])
+        mesh = sloth.SimpleMesh.from_mesh(m)
+        Rs = [meshgen.rotation_euler(0.0, 0.7 * f, 0.2) for f in range(F)]
+        frames = [sloth.FrameSpec(W, H, U, True) for _ in range(F)]
+        b = sloth.Batch(frames, [sloth.DrawSpec(f, mesh, Rs[f]) for f in range(F)])
+        say("case batch on a caller's stream")
+        for _ in range(3):
+            b.run(s.value)
+        _check_frames(b, m, frames, Rs, W, winners=False)
 
-    say("case context on a caller's stream")
-    ctx = sloth.Context(W, H, image=True)
-    ctx.utransform = U
-    ctx.clear()
-    sloth.draw_mesh(ctx, mesh, Rs[3])
-    ctx.render(s.cuda_stream)
-    cells, z, _ = ctx.read()
-    ref = oracle_render(W, H, U, [(m.xyz, m.rgb, Rs[3], None)])
-    assert_frame_equal(cells, z, None, ref, W)
+        say("case context on a caller's stream")
+        ctx = sloth.Context(W, H, image=True)
+        ctx.utransform = U
+        ctx.clear()
+        sloth.draw_mesh(ctx, mesh, Rs[3])
+        ctx.render(s.value)
+        cells, z, _ = ctx.read()
+        ref = oracle_render(W, H, U, [(m.xyz, m.rgb, Rs[3], None)])
+        assert_frame_equal(cells, z, None, ref, W)
+        del b, ctx
+    finally:
+        hip.hipStreamDestroy(s)
