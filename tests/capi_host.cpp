@@ -6,13 +6,24 @@
 //
 //   capi_host <spec> <out>
 //
-// spec (little-endian): u32 kind (0 = triangle soup, 1 = model file),
+// spec (little-endian): u32 kind (0 = triangle soup, 1 = model file,
+//   2 = differentiable shading of a batch of views),
 //   u32 W, u32 H, u32 image, u32 encode_mode, f32 transform[16] (column-major),
 //   kind 0: f32 utransform[16], u32 n, f32 positions[n][3][3], u8 colors[n][3]
 //   kind 1: u32 path_len, char path[path_len]   (utransform from
 //           sloth_context_update over the file's meshes)
-// out: u32 status, u64 n_cells, u32 cells[n_cells], f32 z[W*H], i32 winners[W*H],
-//   f32 utransform[16], u64 n_bytes, u8 encoded[n_bytes]
+//   kind 2 (shading): f32 utransform[16], u32 n_views, f32 view transforms
+//           [n_views][16], u32 n, f32 positions[n][3][3], u8 colors[n][3],
+//           u32 n_verts, u32 faces[n][3], f32 vcolor[n_verts][3],
+//           f32 vnormal[n_verts][3], f32 light[3], f32 target_vcolor[n_verts][3],
+//           f32 target_light[3]  (the transform in the header is unused)
+// out (kinds 0, 1): u32 status, u64 n_cells, u32 cells[n_cells], f32 z[W*H],
+//   i32 winners[W*H], f32 utransform[16], u64 n_bytes, u8 encoded[n_bytes]
+// out (kind 2): u32 status, u64 n_slots, i32 winners[n_slots] (per view),
+//   f32 target[n_slots][3], f64 loss, f32 d_vcolor[n_verts][3],
+//   f32 d_vnormal[n_verts][3], f32 d_light[3] (sloth_shading_forward +
+//   sloth_shading_backward), then f64 loss and the three gradients again
+//   from sloth_shading_step
 // On an error the status is the library's code and sloth_last_error() goes
 // to stderr; nothing else is written.
 #include <cstdint>
@@ -39,6 +50,83 @@ int fail(FILE* out, int rc, const char* what)
     return 1;
 }
 
+// The forward / backward pair and the one-pass step of the differentiable
+// shading over a batch of views of one mesh, as a binding would drive them.
+int shading(Reader& in, FILE* out, uint32_t W, uint32_t H, uint32_t image)
+{
+    float U[16];
+    uint32_t V, n, nv;
+    if (!in.get(U, 16) || !in.get(&V)) return fail(out, SLOTH_EINVAL, "spec");
+    std::vector<float> T((size_t)V * 16);
+    if (!in.get(T.data(), T.size()) || !in.get(&n)) return fail(out, SLOTH_EINVAL, "spec");
+    std::vector<float> pos((size_t)n * 9);
+    std::vector<uint8_t> rgb((size_t)n * 3);
+    std::vector<uint32_t> faces((size_t)n * 3);
+    if (!in.get(pos.data(), pos.size()) || !in.get(rgb.data(), rgb.size()) || !in.get(&nv) ||
+        !in.get(faces.data(), faces.size()))
+        return fail(out, SLOTH_EINVAL, "spec");
+    std::vector<float> vc((size_t)nv * 3), vn((size_t)nv * 3), vct((size_t)nv * 3);
+    float L[3], Lt[3];
+    if (!in.get(vc.data(), vc.size()) || !in.get(vn.data(), vn.size()) || !in.get(L, 3) ||
+        !in.get(vct.data(), vct.size()) || !in.get(Lt, 3))
+        return fail(out, SLOTH_EINVAL, "spec");
+
+    int rc;
+    sloth_mesh* mesh = nullptr;
+    if ((rc = sloth_mesh_create(pos.data(), rgb.data(), n, &mesh))) return fail(out, rc, "sloth_mesh_create");
+    std::vector<sloth_frame_desc> frames(V);
+    std::vector<sloth_draw_desc> draws(V);
+    for (uint32_t v = 0; v < V; ++v) {
+        frames[v].width = W;
+        frames[v].height = H;
+        frames[v].image = (int32_t)image;
+        memcpy(frames[v].utransform, U, sizeof U);
+        draws[v].frame = v;
+        draws[v].mesh = mesh;
+        memcpy(draws[v].transform, &T[(size_t)v * 16], 64);
+    }
+    sloth_batch* b = nullptr;
+    if ((rc = sloth_batch_create(frames.data(), V, draws.data(), V, nullptr, &b))) return fail(out, rc, "sloth_batch_create");
+    sloth_shading* sh = nullptr;
+    if ((rc = sloth_shading_create(b, faces.data(), n, nv, &sh))) return fail(out, rc, "sloth_shading_create");
+    uint64_t slots = 0;
+    if ((rc = sloth_shading_slots(sh, &slots))) return fail(out, rc, "sloth_shading_slots");
+    std::vector<int32_t> win(slots);
+    for (uint32_t v = 0; v < V; ++v)
+        if ((rc = sloth_batch_read_frame(b, v, nullptr, nullptr, win.data() + (size_t)v * W * H)))
+            return fail(out, rc, "sloth_batch_read_frame");
+    // the target: the forward image at the target's parameters
+    std::vector<float> target(slots * 3);
+    if ((rc = sloth_shading_forward(sh, vct.data(), vn.data(), Lt, target.data(), nullptr)))
+        return fail(out, rc, "sloth_shading_forward (target)");
+    if ((rc = sloth_shading_set_target(sh, target.data()))) return fail(out, rc, "sloth_shading_set_target");
+    double loss = 0.0, loss_s = 0.0;
+    std::vector<float> dc(vc.size()), dn(vn.size()), dc_s(vc.size()), dn_s(vn.size());
+    float dl[3], dl_s[3];
+    if ((rc = sloth_shading_forward(sh, vc.data(), vn.data(), L, nullptr, &loss))) return fail(out, rc, "sloth_shading_forward");
+    if ((rc = sloth_shading_backward(sh, dc.data(), dn.data(), dl))) return fail(out, rc, "sloth_shading_backward");
+    if ((rc = sloth_shading_step(sh, vc.data(), vn.data(), L, &loss_s, dc_s.data(), dn_s.data(), dl_s)))
+        return fail(out, rc, "sloth_shading_step");
+
+    const uint32_t ok = 0;
+    fwrite(&ok, 4, 1, out);
+    fwrite(&slots, 8, 1, out);
+    fwrite(win.data(), 4, slots, out);
+    fwrite(target.data(), 4, target.size(), out);
+    fwrite(&loss, 8, 1, out);
+    fwrite(dc.data(), 4, dc.size(), out);
+    fwrite(dn.data(), 4, dn.size(), out);
+    fwrite(dl, 4, 3, out);
+    fwrite(&loss_s, 8, 1, out);
+    fwrite(dc_s.data(), 4, dc_s.size(), out);
+    fwrite(dn_s.data(), 4, dn_s.size(), out);
+    fwrite(dl_s, 4, 3, out);
+    sloth_shading_destroy(sh);
+    sloth_batch_destroy(b);
+    sloth_mesh_destroy(mesh);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -63,6 +151,12 @@ int main(int argc, char** argv)
 
     int rc = sloth_init(0);
     if (rc) return fail(out, rc, "sloth_init");
+    if (kind == 2) {
+        rc = shading(in, out, W, H, image);
+        fclose(fs);
+        fclose(out);
+        return rc;
+    }
     sloth_context* ctx = nullptr;
     if ((rc = sloth_context_create(W, H, (int)image, &ctx))) return fail(out, rc, "sloth_context_create");
 

@@ -105,3 +105,65 @@ def test_model_file_frame_and_flush(tmp_path, sloth, name, W, H, image, mode):
     ref = oracle_render(W, H, U, [(pos, rgb, T, None) for pos, rgb, _, _ in arrays], image=image)
     assert_frame_equal(cells, z, win, ref, W)
     assert enc == oracle.encode(ref.cells, mode, image=image)
+
+
+def test_shading_pair_from_host(tmp_path):
+    """The differentiable-shading forward/backward pair and the one-pass step
+    driven through the bare header (a binding's calls): 3 views of a
+    48 x 24 UV sphere at 128x96, target from colours + N(0, 0.1^2) and the
+    light rotated 10 degrees.  Visibility bit for bit, loss and gradients
+    against the FP64 CPU adjoint to 1e-5 norm-relative."""
+    from sloth_amd import meshgen
+
+    wl = meshgen.workload_shading_views(n_views=3, size=128, n_lon=48, n_lat=24, seed=13)
+    m, W, H, U, Rs = wl["mesh"], wl["W"], 96, wl["U"], wl["Rs"]
+    U = meshgen.screen_transform(W, H, [m.bbox_max()])
+    V, nv = len(Rs), m.verts.shape[0]
+    say(f"case C++ host: shading pair, {V} views {W}x{H}, {m.n} tris")
+    f32 = lambda a: np.ascontiguousarray(a, np.float32).tobytes()   # noqa: E731
+    spec = (_header(2, W, H, True, 0, np.eye(4, dtype=np.float32)) + f32(U) + struct.pack("<I", V)
+            + b"".join(f32(R) for R in Rs) + struct.pack("<I", m.n) + f32(m.xyz)
+            + np.ascontiguousarray(m.rgb, np.uint8).tobytes() + struct.pack("<I", nv)
+            + np.ascontiguousarray(m.faces, np.uint32).tobytes() + f32(wl["vcolor"]) + f32(wl["vnormal"])
+            + f32(wl["light"]) + f32(wl["vcolor_target"]) + f32(wl["light_target"]))
+    sp, op = tmp_path / "spec.bin", tmp_path / "out.bin"
+    sp.write_bytes(spec)
+    r = subprocess.run([HOST, str(sp), str(op)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    b = op.read_bytes()
+    (st,) = struct.unpack_from("<I", b, 0)
+    assert st == 0, st
+    (ns,) = struct.unpack_from("<Q", b, 4)
+    assert ns == V * W * H
+    o = 12
+    win = np.frombuffer(b, np.int32, ns, o)
+    o += 4 * ns
+    target = np.frombuffer(b, np.float32, 3 * ns, o).reshape(ns, 3)
+    o += 12 * ns
+    got = []
+    for _ in range(2):
+        (loss,) = struct.unpack_from("<d", b, o)
+        o += 8
+        dc = np.frombuffer(b, np.float32, 3 * nv, o).reshape(nv, 3)
+        o += 12 * nv
+        dn = np.frombuffer(b, np.float32, 3 * nv, o).reshape(nv, 3)
+        o += 12 * nv
+        dl = np.frombuffer(b, np.float32, 3, o)
+        o += 12
+        got.append((loss, dc, dn, dl))
+    assert o == len(b)
+
+    nz = W * H
+    loss_cpu, gc, gn, gl = 0.0, 0.0, 0.0, 0.0
+    for f, R in enumerate(Rs):
+        fr = oracle.Frame(W, H, True)
+        fr.draw_mesh(U, m.xyz, m.rgb, R)
+        assert np.array_equal(win[f * nz:(f + 1) * nz], fr.win), f
+        lo, _, (a, c, d) = oracle.shade_f64(fr, oracle.mat4_mul(U, R), m.xyz, m.faces, wl["vcolor"],
+                                           wl["vnormal"], wl["light"], target[f * nz:(f + 1) * nz],
+                                           want_grad=True)
+        loss_cpu, gc, gn, gl = loss_cpu + lo, gc + a, gn + c, gl + d
+    for loss, dc, dn, dl in got:
+        assert abs(loss - loss_cpu) <= 1e-5 * abs(loss_cpu)
+        for g, ref in ((dc, gc), (dn, gn), (dl, gl)):
+            assert np.linalg.norm(g.astype(np.float64) - ref) <= 1e-5 * np.linalg.norm(ref)
