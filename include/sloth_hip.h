@@ -309,6 +309,24 @@ int sloth_shading_step(sloth_shading *s, const float *vcolor, const float *vnorm
 int sloth_shading_time(sloth_shading *s, uint32_t steps, uint32_t warmup, float *ms);
 int sloth_shading_destroy(sloth_shading *s);
 
+/* ---- rendering without allocation -------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Renders one frame of one draw -- Context::clear,
+ * draw_mesh(mesh, transform, shader) with the given utransform
+ * (rasterizer.rs:39-93), as a one-draw context renders it -- from device
+ * positions / colours into the caller's device buffers d_cells (W*H (+H in
+ * image mode) cells) and d_z (W*H floats), on `stream` (NULL = the null
+ * stream), without waiting.  It calls no allocator: its working memory is a
+ * fixed device array of the library, one per GPU, 33 MiB, held until the
+ * process ends.  W*H is limited to 4,194,304 z slots (2560 x 1440 fits;
+ * larger frames return SLOTH_EINVAL).  Every call on a GPU uses that GPU's
+ * one array: calls that overlap on one GPU (two streams, two threads) race
+ * and must be serialised by the caller.  Any width, odd included. */
+int sloth_render_frame_noalloc(const float *d_positions, const uint8_t *d_colors, uint32_t n, uint32_t width,
+                               uint32_t height, int image, const float *utransform16, const float *transform16,
+                               const sloth_ramp *shader, uint32_t *d_cells, float *d_z, void *stream);
+
 /* ---- helpers ---------------------------------------------------------- */
 
 /* Context::update's utransform for a size and bounding-box maxima

@@ -568,7 +568,7 @@ int sloth_init(int device)
         return fail(SLOTH_ENODEV, "device %d is %s, this library is built for gfx950", device, p.gcnArchName);
     HIP_TRY(hipSetDevice(device));
     g_device = device;
-    return SLOTH_OK;
+    return noalloc_init(device);
 }
 
 static int ensure_device()
@@ -1604,6 +1604,140 @@ int sloth_shading_destroy(sloth_shading* sh)
 {
     if (sh && sh->b && sh->b->stream) (void)hipStreamSynchronize(sh->b->stream);
     delete sh;
+    return SLOTH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// rendering without allocation
+// ---------------------------------------------------------------------------
+// sloth_render_frame_noalloc renders one single-draw frame on the general
+// (global-key) path, which needs device memory beyond its inputs and
+// outputs: the frame, draw and shader tables the kernels read, one 64-bit
+// key per z slot and one newline entry per row.  Here that memory is a
+// file-scope device array, one copy per device, held until the process
+// ends, so the entry point calls no allocator.  Host code never passes the
+// array's name to a kernel or a copy: it uses the device address that
+// hipGetSymbolAddress gives for the current device.
+namespace {
+constexpr uint64_t NOALLOC_TABLE_BYTES = 1024;                   // Frame at 0, Draw at 256, Ramp at 512
+constexpr uint64_t NOALLOC_MAX_SLOTS = 1ull << 22;               // W*H <= 4,194,304 (2560 x 1440 fits)
+constexpr uint64_t NOALLOC_KEY_BYTES = 8 * NOALLOC_MAX_SLOTS;    // 32 MiB
+constexpr uint64_t NOALLOC_ROW_BYTES = 4 * 65536;                // H <= 65535
+constexpr uint64_t NOALLOC_BYTES = NOALLOC_TABLE_BYTES + NOALLOC_KEY_BYTES + NOALLOC_ROW_BYTES;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static_assert(sizeof(Frame) <= 256 && sizeof(Draw) <= 256 && sizeof(Ramp) <= 256, "table slots");
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// Device address of `scratch` on the current device `dev`, looked up once.
+static int noalloc_scratch(int dev, uint8_t** out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(SLOTH_ENODEV, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(SLOTH_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (uint8_t*)g_scratch_ptr[dev];
+    return SLOTH_OK;
+}
+
+// Sets up the scratch address of the device sloth_init selects.
+int sloth::noalloc_init(int dev)
+{
+    uint8_t* p = nullptr;
+    return noalloc_scratch(dev, &p);
+}
+
+// Writes the launch's tables into scratch from kernel arguments (no host
+// buffer has to outlive the call).
+__global__ void k_noalloc_tables(Frame f, Draw d, Ramp r, uint8_t* base)
+{
+    if (threadIdx.x == 0) *(Frame*)base = f;
+    if (threadIdx.x == 1) *(Draw*)(base + 256) = d;
+    if (threadIdx.x == 2) *(Ramp*)(base + 512) = r;
+}
+
+extern "C" {
+
+int sloth_render_frame_noalloc(const float* d_positions, const uint8_t* d_colors, uint32_t n, uint32_t width,
+                               uint32_t height, int image, const float* utransform16, const float* transform16,
+                               const sloth_ramp* shader, uint32_t* d_cells, float* d_z, void* stream)
+{
+    ARG_CHECK(n == 0 || (d_positions && d_colors), "null positions or colours");
+    ARG_CHECK(utransform16 && transform16 && d_cells && d_z, "null argument");
+    ARG_CHECK(width >= 1 && width <= 65535 && height >= 1 && height <= 65535,
+              "width and height must be in 1..65535");
+    ARG_CHECK(n < (1u << 31) - 1, "a frame holds fewer than 2^31 - 1 triangles");
+    const uint64_t WH = (uint64_t)width * height;
+    if (WH > NOALLOC_MAX_SLOTS)
+        return fail(SLOTH_EINVAL,
+                    "sloth_render_frame_noalloc: %ux%u has %llu z slots; its fixed scratch holds at most %llu "
+                    "(use a context or a batch for larger frames)",
+                    width, height, (unsigned long long)WH, (unsigned long long)NOALLOC_MAX_SLOTS);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    uint8_t* base = nullptr;
+    int rc = noalloc_scratch(dev, &base);
+    if (rc) return rc;
+    Ramp r;
+    if ((rc = to_ramp(shader, r))) return rc;
+
+    Frame f;
+    memset(&f, 0, sizeof f);
+    f.W = width;
+    f.H = height;
+    f.image = image ? 1u : 0u;
+    f.draw_begin = 0;
+    f.draw_end = 1;
+    f.n_tris = n;
+    Draw d;
+    memset(&d, 0, sizeof d);
+    d.xyz = d_positions;
+    d.rgb = d_colors;
+    d.n = n;
+    d.ramp = 0;
+    mat4_mul(utransform16, transform16, d.M);
+
+    hipStream_t s = (hipStream_t)stream;
+    const Frame* d_frame = (const Frame*)base;
+    const Draw* d_draw = (const Draw*)(base + 256);
+    const Ramp* d_ramp = (const Ramp*)(base + 512);
+    uint64_t* keys = (uint64_t*)(base + NOALLOC_TABLE_BYTES);
+    int32_t* nl = (int32_t*)(base + NOALLOC_TABLE_BYTES + NOALLOC_KEY_BYTES);
+    const uint64_t cells = frame_cells(f);
+
+    hipLaunchKernelGGL(k_noalloc_tables, dim3(1), dim3(64), 0, s, f, d, r, base);
+    HIP_TRY(hipGetLastError());
+    {
+        // every key and newline entry of this frame starts empty: the scratch
+        // holds whatever the previous call on this device left
+        const unsigned g = std::max(1u, std::min<unsigned>(Job::blocks_for(std::max<uint64_t>(WH, height)), 8192u));
+        hipLaunchKernelGGL(k_clear_global, dim3(g), dim3(256), 0, s, keys, WH, nl, (uint64_t)height);
+        HIP_TRY(hipGetLastError());
+    }
+    if (n) {
+        hipLaunchKernelGGL(k_raster_global, dim3(Job::blocks_for(n), 1), dim3(256), 0, s, d_draw, 0u, d_frame, keys,
+                           nl);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_winner_global, dim3(Job::blocks_for(WH), 1), dim3(256), 0, s, d_draw, d_ramp, d_frame, 0u,
+                       keys, d_z, (int32_t*)nullptr, 0u);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_resolve_global, dim3(Job::blocks_for(cells), 1), dim3(256), 0, s, d_frame, 0u,
+                       (const uint64_t*)keys, (const int32_t*)nl, d_cells);
+    HIP_TRY(hipGetLastError());
     return SLOTH_OK;
 }
 

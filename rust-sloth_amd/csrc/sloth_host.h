@@ -11,6 +11,10 @@ namespace sloth {
 // Records a message for sloth_last_error() and returns `code`.
 int fail(int code, const char* fmt, ...);
 
+// Looks up the current device's address of the no-allocation entry point's
+// scratch array (once per device).
+int noalloc_init(int dev);
+
 // Device allocation owned by a handle; grow-only, contents not preserved.
 struct DevBuf {
     void* p = nullptr;

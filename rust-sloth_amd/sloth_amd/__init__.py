@@ -119,6 +119,8 @@ SIGNATURES = {
     "sloth_shading_step": (C.c_int, [_vp, _fp, _fp, _fp, C.POINTER(C.c_double), _fp, _fp, _fp]),
     "sloth_shading_time": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _fp]),
     "sloth_shading_destroy": (C.c_int, [_vp]),
+    "sloth_render_frame_noalloc": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _fp, _fp,
+                                             C.POINTER(sloth_ramp), _vp, _vp, _vp]),
     "sloth_screen_transform": (C.c_int, [C.c_uint32, C.c_uint32, _fp, C.c_uint32, _fp]),
     "sloth_mat4_mul": (C.c_int, [_fp, _fp, _fp]),
 }
@@ -412,6 +414,21 @@ class Context:
 def draw_mesh(context: Context, mesh: SimpleMesh, transform, shader: Shader = default_shader) -> None:
     """rasterizer.rs:39-46."""
     context.draw(mesh, transform, shader)
+
+
+def render_frame_noalloc(d_positions: int, d_colors: int, n: int, width: int, height: int, image: bool,
+                         utransform, transform, d_cells: int, d_z: int, shader: Shader = default_shader,
+                         stream: int | None = None) -> None:
+    """One frame of one draw from device arrays into device buffers, with
+    no device allocation (sloth_render_frame_noalloc); returns without
+    waiting.  Pointers are device addresses (ints)."""
+    _ensure_init()
+    u = np.ascontiguousarray(utransform, np.float32).reshape(16)
+    t = np.ascontiguousarray(transform, np.float32).reshape(16)
+    r = shader.to_c()
+    _check(load_library().sloth_render_frame_noalloc(
+        C.c_void_p(d_positions), C.c_void_p(d_colors), n, width, height, int(bool(image)), _fptr(u), _fptr(t),
+        C.byref(r), C.c_void_p(d_cells), C.c_void_p(d_z), C.c_void_p(stream or 0)))
 
 
 def screen_transform(width: int, height: int, bbox_maxes) -> np.ndarray:
