@@ -16,11 +16,12 @@
 //   k_careful      one wavefront per careful triangle: exact box from row
 //                  spans, then its tile entries
 //   k_tile_raster  one 256-thread workgroup per tile, grid (tile x, tile y,
-//                  frame): each list entry's transformed triangle goes to an
-//                  LDS row (found by a hash of its index), its fragments do a
-//                  64-bit LDS atomicMin of the packed keys, then every slot
-//                  re-evaluates its winner from the LDS row and stores its two
-//                  cells and depths (coalesced, streaming stores).  Lists
+//                  frame): each list entry's transformed triangle goes to the
+//                  LDS row of its lane, its fragments do a 64-bit LDS
+//                  atomicMin of packed keys that also name that row, then
+//                  every slot re-evaluates its winner from the row its key
+//                  names and stores its two cells and depths (coalesced,
+//                  streaming stores).  Lists
 //                  longer than the 256 LDS rows run in chunks, storing winners
 //                  after each chunk.  The image-mode tail cells are written by
 //                  the frame's first tile.  When many boxes are big, a variant
@@ -42,9 +43,10 @@
 namespace sloth {
 
 constexpr uint32_t MAX_GROUPS = 8;        // frame groups of a pipelined launch
-// Frames on the tiled path hold fewer triangles than this: the raster's hash
-// table keeps a triangle index and its 8-bit cache row in one 32-bit word.
-constexpr uint32_t TILED_MAX_TRIS = (1u << 24) - 1;
+// Frames on the tiled path hold fewer triangles than this: the raster's
+// 64-bit LDS keys carry the triangle index, the pixel's `sub` bit, a stored
+// flag and the entry's 8-bit cache row in their low 32 bits.
+constexpr uint32_t TILED_MAX_TRIS = (1u << 22) - 1;
 constexpr uint32_t PIPE_MIN_FRAMES = 16;  // frames per group at least
 
 struct TiledPlan {

@@ -459,11 +459,26 @@ __global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws,
 // k_tile_raster
 // ---------------------------------------------------------------------------
 constexpr uint32_t CHUNK = RASTER_BLOCK;   // list entries whose triangles sit in LDS at once (one per thread)
-constexpr uint32_t HASH_CAP = 512;         // open-addressing table: triangle << 8 | cache row
 constexpr uint32_t CACHE_WORDS = 12;
 constexpr uint32_t LDS_RAMPS = 4;          // shaders staged in LDS per tile
 constexpr uint32_t SPAN_MIN_W = 6;         // box width (pixels) from which a wavefront walks row spans
-static_assert(CHUNK <= 256, "hash rows hold a cache row index in one byte");
+static_assert(CHUNK <= 256, "tile keys hold a cache row index in one byte");
+
+// A tile's z-test key: (depth code, frame triangle, sub) decide the order
+// exactly as make_key's do; below them a `stored` flag and the cache row p
+// of the entry that wrote the key (its lane in the chunk).  Two fragments
+// with equal (depth code, triangle, sub) are the same fragment, so the low
+// bits never change a comparison, and the resolve finds the winner's
+// transformed triangle from the key alone.  `stored` is set on keys whose
+// winner a multi-chunk tile has already stored.
+constexpr uint64_t TILE_KEY_STORED = 1u << 8;
+__device__ __forceinline__ uint64_t make_tile_key(float z, uint32_t tri, uint32_t sub, uint32_t p)
+{
+    return ((uint64_t)depth_code(z) << 32) | (uint64_t)((tri << 10) | (sub << 9) | p);
+}
+__device__ __forceinline__ uint32_t tile_key_tri(uint64_t k) { return (uint32_t)k >> 10; }
+__device__ __forceinline__ uint32_t tile_key_sub(uint64_t k) { return ((uint32_t)k >> 9) & 1u; }
+__device__ __forceinline__ uint32_t tile_key_row(uint64_t k) { return (uint32_t)k & 0xFFu; }
 
 __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws, const Frame& fr, uint32_t tri)
 {
@@ -474,8 +489,6 @@ __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws,
     }
     return draws[lo];
 }
-
-__device__ __forceinline__ uint32_t hash_tri(uint32_t T) { return (T * 0x9E3779B1u) >> 23; }   // 9 bits
 
 struct TileGeom {
     int tc0, tc1, tr0, tr1;   // cell columns [tc0,tc1), rows [tr0,tr1)
@@ -580,13 +593,13 @@ __device__ __forceinline__ uint32_t entry_tri(uint32_t e, const uint4* __restric
 // smaller frame's tile grid return at once.
 //
 // The tile's list is taken CHUNK entries at a time: each entry's transformed
-// triangle goes to an LDS row found through a hash of its triangle index, and
-// its fragments take a 64-bit LDS atomicMin of the packed (depth, triangle,
-// sub) keys.  With one chunk, every slot then re-evaluates its winner from
-// the LDS row and stores it.  With several, after each chunk the slots whose
-// key belongs to one of the chunk's triangles store that winner (the last
-// store of a slot is its final winner's, since a later winner comes from a
-// later chunk), and the empty slots are stored at the end.
+// triangle goes to the LDS row of its lane, and its fragments take a 64-bit
+// LDS atomicMin of packed (depth, triangle, sub, row) keys (make_tile_key).
+// With one chunk, every slot then re-evaluates its winner from the row its
+// key names and stores it.  With several, after each chunk the slots whose
+// key was written in that chunk store that winner and flag the key as stored
+// (the last store of a slot is its final winner's, since a later winner
+// comes from a later chunk), and the empty slots are stored at the end.
 //
 // BIG: an entry whose box covers more than BIG_BOX of the tile's pixels in
 // its direct region is not walked by its own lane; after its own walk, each
@@ -645,7 +658,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         if (k != KEY_EMPTY) {
             c0 = c1 = cell;
             z0 = z;
-            wt = (int32_t)((uint32_t)k >> 1);
+            wt = (int32_t)tile_key_tri(k);
         }
         if (fr.image && my_col == 0) {
             // cell row*W + 1: the newline of row `row` comes after every pixel
@@ -660,8 +673,8 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         __builtin_nontemporal_store((uint64_t)__float_as_uint(z0) | ((uint64_t)__float_as_uint(F32_MAX) << 32),
                                     (uint64_t*)(fz + id));
         if (win) {
-            // win_raw keeps the key's low word (triangle << 1 | sub) for the shading pass
-            const int32_t wv = (k == KEY_EMPTY) ? -1 : (win_raw ? (int32_t)(uint32_t)k : wt);
+            // win_raw gives triangle << 1 | sub for the shading pass
+            const int32_t wv = (k == KEY_EMPTY) ? -1 : (win_raw ? (int32_t)((uint32_t)wt << 1 | tile_key_sub(k)) : wt);
             __builtin_nontemporal_store((uint64_t)(uint32_t)wv | ((uint64_t)0xFFFFFFFFu << 32),
                                         (uint64_t*)(win + fr.z_off + id));
         }
@@ -680,23 +693,15 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             return ramp_glyph(ramps[r], sh);
         };
         uint64_t* keys = smem;
-        uint32_t* htag = (uint32_t*)(keys + S);   // T << 8 | cache row (< CHUNK), or EMPTY32
-        float* cache = (float*)(htag + HASH_CAP);
+        float* cache = (float*)(keys + S);
         const bool one_draw = fr.draw_end - fr.draw_begin == 1;
         const bool multi = n > CHUNK;
 
-        // Winner of slot s with key k, re-evaluated from its cache row; false
-        // when the key's triangle is not in the current chunk.
-        auto winner = [&](int row, uint64_t k, uint32_t& cell, float& z) -> bool {
-            const uint32_t T = (uint32_t)k >> 1, sub = (uint32_t)k & 1u;
-            uint32_t h = hash_tri(T), tag;
-            for (uint32_t probe = 0; probe < HASH_CAP; ++probe) {
-                tag = htag[h];
-                if ((tag >> 8) == T || tag == EMPTY32) break;
-                h = (h + 1) & (HASH_CAP - 1);
-            }
-            if (tag == EMPTY32 || (tag >> 8) != T) return false;
-            const float4* row4 = (const float4*)(cache + (tag & 0xFFu) * CACHE_WORDS);
+        // Winner of a slot with key k (written in the current chunk),
+        // re-evaluated from the cache row the key names.
+        auto winner = [&](int row, uint64_t k, uint32_t& cell, float& z) {
+            const uint32_t sub = tile_key_sub(k);
+            const float4* row4 = (const float4*)(cache + tile_key_row(k) * CACHE_WORDS);
             const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
             const int col = my_col;
             const float px = (float)(sub ? (col >> 1) : ((col + g.W) >> 1));
@@ -710,14 +715,11 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             const float sh = q2.z * (w0 + w1 + w2);
             const uint32_t cw = __float_as_uint(q2.w);
             cell = glyph_of(cw & 0xFFu, sh) | (cw & 0xFFFFFF00u);
-            return true;
         };
 
         for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
         for (uint32_t base = 0; base < n; base += CHUNK) {
-            if (base) __syncthreads();   // the previous chunk's winner stores no longer read its rows
-            for (uint32_t h = threadIdx.x; h < HASH_CAP; h += blockDim.x) htag[h] = EMPTY32;
-            __syncthreads();
+            __syncthreads();   // keys cleared / the previous chunk's winner stores no longer read its rows
             const uint32_t p = threadIdx.x;
             bool big = false;              // BIG: this lane's box is walked by its wavefront
             uint32_t big_T = 0, big_rect = 0;
@@ -736,11 +738,6 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 row4[0] = make_float4(t.x1, t.y1, t.x2, t.y2);
                 row4[1] = make_float4(t.x3, t.y3, t.z1, t.z2 - t.z1);
                 row4[2] = make_float4(t.z3 - t.z1, t.a, na, __uint_as_float(cw));
-                uint32_t h = hash_tri(T);
-                for (uint32_t probe = 0; probe < HASH_CAP; ++probe) {
-                    if (atomicCAS(&htag[h], EMPTY32, (T << 8) | p) == EMPTY32) break;
-                    h = (h + 1) & (HASH_CAP - 1);
-                }
                 RowSpans spans;
                 if (!given) {
                     box_x(t, tight, bx0, bx1);
@@ -761,7 +758,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                         edges(t, px, py, w0, w1, w2);
                         const float z = depth(t, w1, w2);
                         if (inside(w0, w1, w2) && z < F32_MAX)
-                            atomicMin((unsigned long long*)&keys[slot], (unsigned long long)make_key(z, T, sub));
+                            atomicMin((unsigned long long*)&keys[slot], (unsigned long long)make_tile_key(z, T, sub, p));
                     };
                     if constexpr (BIG) {
                         int xa, xb, ya, yb;
@@ -796,7 +793,8 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                     const int l = __builtin_ctzll(m);
                     m &= m - 1;
                     const uint32_t T = __builtin_amdgcn_readlane(big_T, l), rc = __builtin_amdgcn_readlane(big_rect, l);
-                    const float4* row4 = (const float4*)(cache + ((threadIdx.x & ~63u) + (uint32_t)l) * CACHE_WORDS);
+                    const uint32_t prow = (threadIdx.x & ~63u) + (uint32_t)l;   // the owner lane's cache row
+                    const float4* row4 = (const float4*)(cache + prow * CACHE_WORDS);
                     const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
                     const float x1 = q0.x, y1 = q0.y, x2 = q0.z, y2 = q0.w, x3 = q1.x, y3 = q1.y;
                     const float A0 = x3 - x2, B0 = y3 - y2, A1 = x1 - x3, B1 = y1 - y3, A2 = x2 - x1, B2 = y2 - y1;
@@ -813,7 +811,8 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                         const float w2 = A2 * (py - y1) - B2 * (px - x1);
                         const float z = z1 + a * (w1 * dz2 + w2 * dz3);
                         if (inside(w0, w1, w2) && z < F32_MAX)
-                            atomicMin((unsigned long long*)&keys[(y << sws) + x], (unsigned long long)make_key(z, T, 1u));
+                            atomicMin((unsigned long long*)&keys[(y << sws) + x],
+                                      (unsigned long long)make_tile_key(z, T, 1u, prow));
                         xx += dx;
                         yy += dy;
                         if (xx >= nx) {
@@ -825,14 +824,18 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             }
             __syncthreads();
             if (multi)
-                // slots whose key now belongs to this chunk store that winner
-                // (a later chunk that changes the key stores again; the same
-                // thread owns slot s throughout, so its last store lands last)
+                // slots whose key was written in this chunk (no stored flag)
+                // store that winner and flag the key; a later chunk that
+                // changes the key stores again.  The same thread owns slot s
+                // throughout, so its last store lands last.
                 for_my_slots([&](uint32_t s, int row, uint64_t id) {
                     const uint64_t k = keys[s];
+                    if (k == KEY_EMPTY || (k & TILE_KEY_STORED)) return;
                     uint32_t cell;
                     float z;
-                    if (k != KEY_EMPTY && winner(row, k, cell, z)) store(row, id, k, cell, z);
+                    winner(row, k, cell, z);
+                    store(row, id, k, cell, z);
+                    keys[s] = k | TILE_KEY_STORED;
                 });
         }
         for_my_slots([&](uint32_t s, int row, uint64_t id) {
@@ -906,9 +909,9 @@ int TiledPlan::set_tile(uint32_t tw, uint32_t th)
 
 size_t TiledPlan::raster_lds() const
 {
-    // keys (8 B per slot), hash words (4 B), triangle rows
+    // keys (8 B per slot), triangle rows
     size_t S = (size_t)(TW / 2) * TH;
-    return S * 8 + (size_t)HASH_CAP * 4 + (size_t)CHUNK * CACHE_WORDS * 4;
+    return S * 8 + (size_t)CHUNK * CACHE_WORDS * 4;
 }
 
 int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
