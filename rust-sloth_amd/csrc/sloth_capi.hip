@@ -263,9 +263,9 @@ struct Job {
         n_z = z;
         n_rows = rows;
         n_items = items;
-        // the tiled path keeps a draw's shader index in 8 bits of an LDS word
-        // and a triangle index with a cache row in a 32-bit hash word
-        path = (all_even && small_frames && ramps.size() <= 256 && !force_global()) ? PATH_TILED : PATH_GLOBAL;
+        // the tiled path keeps a draw's shader index in the low byte of an
+        // LDS word, where 0xFF marks a triangle whose glyph is constant
+        path = (all_even && small_frames && ramps.size() < 256 && !force_global()) ? PATH_TILED : PATH_GLOBAL;
         int rc;
         if ((rc = d_frames.reserve(sizeof(Frame) * std::max<size_t>(1, frames.size())))) return rc;
         if ((rc = d_draws.reserve(sizeof(Draw) * std::max<size_t>(1, draws.size())))) return rc;

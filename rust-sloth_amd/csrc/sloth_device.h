@@ -177,6 +177,19 @@ __device__ __forceinline__ float shade_value(float nzu, float a, float w0, float
     return nzu * a * (w0 + w1 + w2);
 }
 
+// Step index of shade s under a shader with non-decreasing thresholds: the
+// first i with s <= thr[i], or n when there is none or s is NaN (the
+// fallback).  The count is the one ramp_glyph uses below.
+__device__ __forceinline__ uint32_t ramp_index_sorted(const Ramp& r, float s)
+{
+    const uint32_t c = (uint32_t)(r.thr[3] < s) + (uint32_t)(r.thr[7] < s) + (uint32_t)(r.thr[11] < s) +
+                       (uint32_t)(r.thr[15] < s);
+    const uint32_t b = 4u * min(c, 3u);
+    const uint32_t i = b + (uint32_t)(r.thr[b] < s) + (uint32_t)(r.thr[b + 1] < s) + (uint32_t)(r.thr[b + 2] < s) +
+                       (uint32_t)(r.thr[b + 3] < s);
+    return (i >= r.n || s != s) ? r.n : i;
+}
+
 __device__ __forceinline__ uint8_t ramp_glyph(const Ramp& r, float s)
 {
     if (r.sorted) {
@@ -209,6 +222,13 @@ __device__ __forceinline__ uint32_t depth_code(float z)
     const uint32_t u = __float_as_uint(z + 0.0f);
     return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
 }
+
+// Inverse of depth_code for every depth but -0 (whose code is +0's).
+__device__ __forceinline__ float depth_of_code(uint32_t code)
+{
+    return __uint_as_float((code & 0x80000000u) ? (code & 0x7FFFFFFFu) : ~code);
+}
+constexpr uint32_t DEPTH_CODE_ZERO = 0x80000000u;   // depth_code(+0) = depth_code(-0)
 
 // Packed z-test key: (depth code, frame triangle index, sub).  `sub` orders the
 // two logical pixels a z slot can receive from one triangle when 2x >= W wraps

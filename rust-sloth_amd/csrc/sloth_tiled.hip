@@ -692,17 +692,63 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             if (r < LDS_RAMPS) return ramp_glyph(s_ramp[r], sh);
             return ramp_glyph(ramps[r], sh);
         };
+        // The glyph a triangle gives every pixel of its box [X0,X1] x [Y0,Y1]
+        // when that is provably one glyph, else ~0u.  The pixel's shade is
+        // fl(na * S) with S = fl(fl(w0 + w1) + w2) (rasterizer.rs:76).  Each
+        // FP32 edge function errs from the exact one by at most
+        // 4.0003u (|A| |py - ay| + |B| |px - ax|) (u = 2^-24), the exact ones
+        // sum to the exact orient, the two additions add at most 2.0001u of
+        // their inputs' magnitudes, and the FP32 orient o errs from the exact
+        // one by at most 4.0002u (|A2| |y3 - y1| + |B2| |x3 - x1|).  So every
+        // S lies within E = 10u (Sb + Q) of o, with Sb the edge terms' sum
+        // over the box and Q o's terms; 10u also covers the rounding of E
+        // and of o -+ E (Sb >= |o|).  fl(na * .) and the step index are
+        // monotone, so equal indices at the two ends mean one glyph for the
+        // whole box.  Shaders that are not non-decreasing, and non-finite
+        // values, take the per-pixel evaluation.
+        auto box_glyph = [&](uint32_t r, const Tri& t, float na, int bx0, int bx1, int by0, int by1) -> uint32_t {
+            const Ramp& rp = r < LDS_RAMPS ? s_ramp[r] : ramps[r];
+            if (!rp.sorted) return ~0u;
+            const float X0 = (float)bx0, X1 = (float)bx1, Y0 = (float)by0, Y1 = (float)by1;
+            const float Sb =
+                fabsf(t.A0) * fmaxf(fabsf(Y0 - t.y2), fabsf(Y1 - t.y2)) + fabsf(t.B0) * fmaxf(fabsf(X0 - t.x2), fabsf(X1 - t.x2)) +
+                fabsf(t.A1) * fmaxf(fabsf(Y0 - t.y3), fabsf(Y1 - t.y3)) + fabsf(t.B1) * fmaxf(fabsf(X0 - t.x3), fabsf(X1 - t.x3)) +
+                fabsf(t.A2) * fmaxf(fabsf(Y0 - t.y1), fabsf(Y1 - t.y1)) + fabsf(t.B2) * fmaxf(fabsf(X0 - t.x1), fabsf(X1 - t.x1));
+            const float ey = t.y3 - t.y1, ex = t.x3 - t.x1;
+            const float o = t.A2 * ey - t.B2 * ex;
+            const float E = 5.9604645e-07f * (Sb + fabsf(t.A2) * fabsf(ey) + fabsf(t.B2) * fabsf(ex));
+            const float s1 = na * (o - E), s2 = na * (o + E);
+            // NaN or infinity anywhere fails the finite test (v_maximum3 propagates NaN)
+            const float mag = __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(na), fabsf(s1)), fabsf(s2));
+            if (!(mag < INFINITY)) return ~0u;
+            const uint32_t i1 = ramp_index_sorted(rp, s1), i2 = ramp_index_sorted(rp, s2);
+            if (i1 != i2) return ~0u;
+            return i1 < rp.n ? rp.glyph[i1] : rp.fallback;
+        };
         uint64_t* keys = smem;
         float* cache = (float*)(keys + S);
         const bool one_draw = fr.draw_end - fr.draw_begin == 1;
         const bool multi = n > CHUNK;
 
-        // Winner of a slot with key k (written in the current chunk),
-        // re-evaluated from the cache row the key names.
+        // Winner of a slot with key k (written in the current chunk), from
+        // the cache row the key names.  A row whose triangle has one glyph
+        // over its box holds the finished cell (low byte of its last word
+        // 0xFF); the depth then comes from the key's code, except for a
+        // zero depth, whose sign the code does not keep.  Other winners are
+        // re-evaluated from the transformed triangle.
         auto winner = [&](int row, uint64_t k, uint32_t& cell, float& z) {
             const uint32_t sub = tile_key_sub(k);
             const float4* row4 = (const float4*)(cache + tile_key_row(k) * CACHE_WORDS);
-            const float4 q0 = row4[0], q1 = row4[1], q2 = row4[2];
+            const float4 q2 = row4[2];
+            const uint32_t cw = __float_as_uint(q2.w);
+            const uint32_t code = (uint32_t)(k >> 32);
+            const bool fast = (cw & 0xFFu) == 0xFFu;
+            if (fast && code != DEPTH_CODE_ZERO) {
+                cell = __float_as_uint(q2.z);
+                z = depth_of_code(code);
+                return;
+            }
+            const float4 q0 = row4[0], q1 = row4[1];
             const int col = my_col;
             const float px = (float)(sub ? (col >> 1) : ((col + g.W) >> 1));
             const float py = (float)(sub ? row : row - 1);
@@ -713,8 +759,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             const float w2 = (x2 - x1) * (py - y1) - (y2 - y1) * (px - x1);
             z = q1.z + q2.y * (w1 * q1.w + w2 * q2.x);
             const float sh = q2.z * (w0 + w1 + w2);
-            const uint32_t cw = __float_as_uint(q2.w);
-            cell = glyph_of(cw & 0xFFu, sh) | (cw & 0xFFFFFF00u);
+            cell = fast ? __float_as_uint(q2.z) : glyph_of(cw & 0xFFu, sh) | (cw & 0xFFFFFF00u);
         };
 
         for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) keys[s] = KEY_EMPTY;
@@ -733,17 +778,22 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
                 const float na = unit_normal_z(t) * t.a;   // rasterizer.rs:76 multiplies these first
                 const auto* c = global_ptr(dr.rgb + (size_t)li * 3);
-                const uint32_t cw = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24) | dr.ramp;
-                float4* row4 = (float4*)(cache + p * CACHE_WORDS);
-                row4[0] = make_float4(t.x1, t.y1, t.x2, t.y2);
-                row4[1] = make_float4(t.x3, t.y3, t.z1, t.z2 - t.z1);
-                row4[2] = make_float4(t.z3 - t.z1, t.a, na, __uint_as_float(cw));
-                RowSpans spans;
+                const uint32_t rgb = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24);
                 if (!given) {
                     box_x(t, tight, bx0, bx1);
                     by0 = t.y0;
                     by1 = t.y1d - 1;
                 }
+                // a triangle with one glyph over its box keeps its finished
+                // cell where the shade factor would be, flagged by 0xFF
+                const uint32_t bg = box_glyph(dr.ramp, t, na, bx0, bx1, by0, by1);
+                const float w2 = bg != ~0u ? __uint_as_float(bg | rgb) : na;
+                const uint32_t w3 = rgb | (bg != ~0u ? 0xFFu : dr.ramp);
+                float4* row4 = (float4*)(cache + p * CACHE_WORDS);
+                row4[0] = make_float4(t.x1, t.y1, t.x2, t.y2);
+                row4[1] = make_float4(t.x3, t.y3, t.z1, t.z2 - t.z1);
+                row4[2] = make_float4(t.z3 - t.z1, t.a, w2, __uint_as_float(w3));
+                RowSpans spans;
                 // row spans cost set-up and per-row work that only boxes of
                 // some width repay; a wavefront walks as long as its widest
                 // box, so it takes them when any lane's box is that wide

@@ -1,0 +1,139 @@
+"""The constant-glyph proof behind the tile raster's resolve (CPU).
+
+For each list entry the raster decides whether the reference's shade
+fl(fl(nzu * a) * fl(fl(w0 + w1) + w2)) (rasterizer.rs:76) picks one glyph of
+the shader at every pixel of the triangle's candidate box; if so, the
+winning slot's cell comes from the entry without re-evaluating the pixel.
+The decision bounds every pixel's edge-function sum by the FP32 orient o
+plus or minus E = 10u (Sb + Q), Sb the sum of |A| |py - ay| + |B| |px - ax|
+over the box's corners and Q the orient's own terms (u = 2^-24).
+
+Here that decision is restated in float32 numpy (one rounding per
+operation, as on the GPU) for triangles whose unit normal z sits within a
+few ulps of default_shader's thresholds, and checked by brute force against
+the reference's per-pixel shade over every pixel of each box: a box whose
+glyph the decision calls constant must show one glyph at every pixel.
+Triangles whose glyph does vary across the box must exist in the sample,
+so the decision is tested where it matters."""
+from __future__ import annotations
+
+import numpy as np
+
+f32 = np.float32
+THR = np.array([0.20, 0.30, 0.40, 0.50, 0.60, 0.70, 0.80, 0.90, 1.0], np.float32)
+
+
+def _index(s):
+    """Step of default_shader (rasterizer.rs:5-27): first i with s <= thr[i];
+    9 (the fallback ' ') when none, NaN included."""
+    s = np.asarray(s, np.float32)
+    i = (THR[None, :] < s[..., None]).sum(-1)
+    return np.where(np.isnan(s), 9, i)
+
+
+def _triangles(n, rng, W=3840, H=2160):
+    """Small front-facing screen-space triangles on planes z = kx x + ky y + c
+    whose unit normal z is a threshold times (1 + N(0, 2e-7)) (plus a share
+    away from every threshold)."""
+    c = np.stack([rng.uniform(60, W / 2 - 60, n), rng.uniform(60, H - 60, n)], 1)
+    d = rng.uniform(-3.2, 3.2, (n, 3, 2))
+    xy = (c[:, None, :] + d).astype(np.float32)
+    e1, e2 = xy[:, 1] - xy[:, 0], xy[:, 2] - xy[:, 0]
+    flip = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0] < 0
+    xy[flip] = xy[flip][:, [0, 2, 1]]
+    near = rng.uniform(0, 1, n) < 0.9
+    target = np.where(near, rng.choice(THR, n) * (1 + rng.normal(0, 2e-7, n)), rng.uniform(0.05, 1.0, n))
+    k = np.sqrt(np.maximum(1.0 / np.minimum(target, 1.0) ** 2 - 1.0, 0.0))
+    ph = rng.uniform(0, 2 * np.pi, n)
+    z = (k * np.cos(ph))[:, None] * xy[..., 0] + (k * np.sin(ph))[:, None] * xy[..., 1] + rng.uniform(-50, 50, (n, 1))
+    return np.concatenate([xy, z.astype(np.float32)[..., None]], 2).astype(np.float32), W, H, near
+
+
+def _setup(t, W, H):
+    """Per-triangle quantities of setup_tri / unit_normal_z / box_x, float32."""
+    x1, y1, z1 = t[:, 0, 0], t[:, 0, 1], t[:, 0, 2]
+    x2, y2, z2 = t[:, 1, 0], t[:, 1, 1], t[:, 1, 2]
+    x3, y3, z3 = t[:, 2, 0], t[:, 2, 1], t[:, 2, 2]
+    A0, B0, A1, B1, A2, B2 = x3 - x2, y3 - y2, x1 - x3, y1 - y3, x2 - x1, y2 - y1
+    o = A2 * (y3 - y1) - B2 * (x3 - x1)
+    a = f32(1) / o
+    e1x, e1y, e1z = x2 - x1, y2 - y1, z2 - z1
+    e2x, e2y, e2z = x3 - x1, y3 - y1, z3 - z1
+    nx = e1y * e2z - e1z * e2y
+    ny = e1z * e2x - e1x * e2z
+    nz = e1x * e2y - e1y * e2x
+    n2 = f32(0) + ((nx * nx + nz * nz) + (ny * ny + f32(0)))
+    na = (nz / np.sqrt(n2)) * a
+    minx, maxx = np.minimum(x1, np.minimum(x2, x3)), np.maximum(x1, np.maximum(x2, x3))
+    miny, maxy = np.minimum(y1, np.minimum(y2, y3)), np.maximum(y1, np.maximum(y2, y3))
+    x0 = np.ceil(np.maximum(minx, f32(1))).astype(np.int64)
+    x1d = np.ceil(np.minimum(maxx * f32(2), f32(W - 1))).astype(np.int64)
+    y0 = np.ceil(np.maximum(miny, f32(1))).astype(np.int64)
+    y1d = np.ceil(np.minimum(maxy, f32(H - 1))).astype(np.int64)
+    bx0 = np.maximum(x0, np.ceil(minx - f32(0.03125)).astype(np.int64))
+    bx1 = np.minimum(x1d - 1, np.floor(maxx + f32(0.03125)).astype(np.int64))
+    return dict(x1=x1, y1=y1, x2=x2, y2=y2, x3=x3, y3=y3, A0=A0, B0=B0, A1=A1, B1=B1, A2=A2, B2=B2, o=o, na=na,
+                bx0=bx0, bx1=bx1, by0=y0, by1=y1d - 1)
+
+
+def _box_glyph_index(s):
+    """The raster's decision (box_glyph in sloth_tiled.hip): the constant step
+    index, or -1."""
+    X0, X1 = s["bx0"].astype(np.float32), s["bx1"].astype(np.float32)
+    Y0, Y1 = s["by0"].astype(np.float32), s["by1"].astype(np.float32)
+
+    def term(A, B, ax, ay):
+        return np.abs(A) * np.maximum(np.abs(Y0 - ay), np.abs(Y1 - ay)) + \
+            np.abs(B) * np.maximum(np.abs(X0 - ax), np.abs(X1 - ax))
+
+    Sb = (((((term(s["A0"], s["B0"], s["x2"], s["y2"])))) + term(s["A1"], s["B1"], s["x3"], s["y3"]))
+          + term(s["A2"], s["B2"], s["x1"], s["y1"]))
+    ey, ex = s["y3"] - s["y1"], s["x3"] - s["x1"]
+    o = s["A2"] * ey - s["B2"] * ex
+    E = f32(5.9604645e-07) * ((Sb + np.abs(s["A2"]) * np.abs(ey)) + np.abs(s["B2"]) * np.abs(ex))
+    with np.errstate(all="ignore"):
+        s1, s2 = s["na"] * (o - E), s["na"] * (o + E)
+        mag = np.maximum(np.abs(s["na"]), np.maximum(np.abs(s1), np.abs(s2)))
+    i1, i2 = _index(s1), _index(s2)
+    ok = np.isfinite(mag) & (i1 == i2)
+    return np.where(ok, i1, -1)
+
+
+def test_constant_glyph_decision_holds_per_pixel():
+    rng = np.random.default_rng(2024)
+    t, W, H, near = _triangles(40000, rng)
+    s = _setup(t, W, H)
+    claim = _box_glyph_index(s)
+    lo_idx = np.full(len(t), 99)
+    hi_idx = np.full(len(t), -1)
+    bw = (s["bx1"] - s["bx0"] + 1).max()
+    bh = (s["by1"] - s["by0"] + 1).max()
+    assert bw <= 10 and bh <= 10
+    with np.errstate(all="ignore"):
+        for dy in range(bh):
+            for dx in range(bw):
+                px = (s["bx0"] + dx).astype(np.float32)
+                py = (s["by0"] + dy).astype(np.float32)
+                inbox = (s["bx0"] + dx <= s["bx1"]) & (s["by0"] + dy <= s["by1"])
+                # rasterizer.rs:71-76, one rounding per operation
+                w0 = s["A0"] * (py - s["y2"]) - s["B0"] * (px - s["x2"])
+                w1 = s["A1"] * (py - s["y3"]) - s["B1"] * (px - s["x3"])
+                w2 = s["A2"] * (py - s["y1"]) - s["B2"] * (px - s["x1"])
+                shade = s["na"] * ((w0 + w1) + w2)
+                idx = _index(shade)
+                lo_idx = np.where(inbox, np.minimum(lo_idx, idx), lo_idx)
+                hi_idx = np.where(inbox, np.maximum(hi_idx, idx), hi_idx)
+    claimed = claim >= 0
+    nonempty = hi_idx >= 0
+    # every claimed box shows exactly its claimed glyph at every pixel
+    bad = claimed & nonempty & ((lo_idx != claim) | (hi_idx != claim))
+    assert not bad.any(), np.nonzero(bad)[0][:10]
+    # the sample holds boxes whose glyph varies (the decision must decline
+    # those); away from the thresholds (shades spread over [0.05, 1]) it
+    # accepts nearly every box, and close to them some constant ones too
+    varies = nonempty & (lo_idx != hi_idx)
+    assert varies.sum() >= 100, varies.sum()
+    assert not (claimed & varies).any()
+    far = nonempty & ~near
+    assert (claimed & far).sum() >= 0.99 * far.sum(), ((claimed & far).sum(), far.sum())
+    assert (claimed & near & nonempty).sum() >= 1000

@@ -464,3 +464,23 @@ def test_model_files_render(sloth):
     ref = oracle_render(W, H, U, [(m.positions, m.colors, R, None) for m in queue])
     assert_frame_equal(cells, z, win, ref, W)
     assert ref.stats[2] > 0
+
+
+def test_near_threshold_shades(sloth):
+    """Small triangles whose unit normal z lies within a few ulps of
+    default_shader's thresholds, so that the glyph can change from pixel to
+    pixel of one triangle: the tile raster takes the cell from the entry only
+    when it has proved the glyph constant over the box, and re-evaluates the
+    pixel otherwise.  Drawn in screen space (identity transforms)."""
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_glyph_bound import _triangles
+
+    rng = np.random.default_rng(77)
+    t, _, _, _ = _triangles(6000, rng, W=1280, H=720)
+    W, H = 1280, 720
+    I = np.eye(4, dtype=np.float32).reshape(16)
+    rgb = rng.integers(0, 256, (len(t), 3), dtype=np.uint8)
+    _check(sloth, "near-threshold shades", W, H, I, [(t, rgb, I, None)])
