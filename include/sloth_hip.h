@@ -254,12 +254,27 @@ int sloth_batch_time(sloth_batch *b, uint32_t steps, uint32_t warmup, int mode, 
  * in 8..128, height in 4..64).  Takes effect at the next render. */
 int sloth_batch_set_tile(sloth_batch *b, uint32_t tile_w, uint32_t tile_h);
 
-/* Renders the batch with each candidate tile size (32..128 x 8..32 cells),
- * timed with HIP events, and keeps the fastest; the choice is remembered for
- * batches of the same class (frame sizes and triangle count) in this process
- * and reused without timing.  Writes the chosen size to tile_w / tile_h (may
- * be NULL). */
+/* Tile raster variant of the batch's next renders: 1 = wavefronts walk the
+ * boxes of more than 128 pixels in a tile together, 0 = every lane walks its
+ * own box, -1 = the choice of the sizing run (boxes of more than 128 pixels
+ * in at least 1 triangle of 512).  Results are identical either way. */
+int sloth_batch_set_walk(sloth_batch *b, int mode);
+
+/* Renders the batch with each candidate tile size -- 8..64 logical pixels
+ * a side, i.e. 16, 32, 64 or 128 cells wide by 8, 16, 32 or 64 rows -- timed
+ * with HIP events over 3 graph renders each, keeps the fastest, then times
+ * the tile raster with and without the wavefront walk of big boxes at that
+ * size and keeps the faster.  The choice is remembered for batches of the
+ * same class (frame sizes, cell count and log2 of the triangle count) in this
+ * process and reused without timing.  Writes the chosen size to tile_w /
+ * tile_h (may be NULL; 0 for batches on the general path, which has no
+ * tiles). */
 int sloth_batch_autotune(sloth_batch *b, uint32_t *tile_w, uint32_t *tile_h);
+
+/* The batch's current raster plan: tile size in cells (0 x 0 on the general
+ * path) and whether the tile raster walks big boxes by wavefront (1), not
+ * (0), or -1 on the general path. */
+int sloth_batch_plan(const sloth_batch *b, uint32_t *tile_w, uint32_t *tile_h, int *big_walk);
 
 int sloth_batch_destroy(sloth_batch *b);
 

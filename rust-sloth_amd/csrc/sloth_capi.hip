@@ -1225,6 +1225,14 @@ int sloth_batch_set_tile(sloth_batch* b, uint32_t tile_w, uint32_t tile_h)
     return SLOTH_OK;
 }
 
+int sloth_batch_set_walk(sloth_batch* b, int mode)
+{
+    ARG_CHECK(b, "null batch");
+    ARG_CHECK(mode >= -1 && mode <= 1, "mode must be -1 (the plan's choice), 0 or 1");
+    b->job.tiled.set_walk(mode);
+    return SLOTH_OK;
+}
+
 }  // extern "C"
 
 // Autotuner cache: class of a batch -> best tile size.
@@ -1232,8 +1240,12 @@ int sloth_batch_set_tile(sloth_batch* b, uint32_t tile_w, uint32_t tile_h)
 #include <mutex>
 #include <tuple>
 
+struct TuneChoice {
+    uint32_t tw, th;
+    int walk;
+};
 static std::mutex g_tune_mu;
-static std::map<std::tuple<uint64_t, uint64_t, uint32_t>, std::pair<uint32_t, uint32_t>> g_tune;
+static std::map<std::tuple<uint64_t, uint64_t, uint32_t>, TuneChoice> g_tune;
 
 static std::tuple<uint64_t, uint64_t, uint32_t> tune_class(const Job& j)
 {
@@ -1262,38 +1274,67 @@ int sloth_batch_autotune(sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h)
         if (tile_h) *tile_h = 0;
         return SLOTH_OK;   // the general path has no tiles
     }
+    auto apply = [&](const TuneChoice& c) -> int {
+        int r = sloth_batch_set_tile(b, c.tw, c.th);
+        if (r) return r;
+        j.tiled.set_walk(c.walk);
+        if (tile_w) *tile_w = c.tw;
+        if (tile_h) *tile_h = c.th;
+        return SLOTH_OK;
+    };
     const auto key = tune_class(j);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
-        if (it != g_tune.end()) {
-            if ((rc = sloth_batch_set_tile(b, it->second.first, it->second.second))) return rc;
-            if (tile_w) *tile_w = it->second.first;
-            if (tile_h) *tile_h = it->second.second;
-            return SLOTH_OK;
-        }
+        if (it != g_tune.end()) return apply(it->second);
     }
-    static const uint32_t cand[][2] = {{64, 16}, {64, 8}, {64, 32}, {128, 8}, {128, 16}, {128, 32},
-                                       {32, 16}, {32, 32}, {32, 8}};
-    float best = 1e30f;
-    uint32_t bw = j.tiled.TW, bh = j.tiled.TH;
-    for (const auto& c : cand) {
-        if ((rc = sloth_batch_set_tile(b, c[0], c[1]))) return rc;
+    // Tiles of 8..64 logical pixels a side (a logical pixel is two cells
+    // wide): 16..128 cells by 8..64 rows.  Each candidate is timed over 3
+    // graph renders after one untimed render (which also sizes the lists).
+    auto time_it = [&](float& ms) -> int {
+        int r = sloth_batch_time(b, 3, 1, 0, &ms);
+        return r;
+    };
+    TuneChoice best{j.tiled.TW, j.tiled.TH, -1};
+    float best_ms = 1e30f;
+    static const uint32_t widths[] = {16, 32, 64, 128}, heights[] = {8, 16, 32, 64};
+    j.tiled.set_walk(-1);
+    for (uint32_t tw : widths)
+        for (uint32_t th : heights) {
+            if ((rc = sloth_batch_set_tile(b, tw, th))) return rc;
+            float ms = 0.0f;
+            if ((rc = time_it(ms))) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = TuneChoice{tw, th, -1};
+            }
+        }
+    // then the raster variant: the sizing run's choice against the other
+    if ((rc = sloth_batch_set_tile(b, best.tw, best.th))) return rc;
+    {
         float ms = 0.0f;
-        if ((rc = sloth_batch_time(b, 2, 1, 0, &ms))) return rc;
-        if (ms < best) {
-            best = ms;
-            bw = c[0];
-            bh = c[1];
-        }
+        if ((rc = time_it(ms))) return rc;   // sizes the lists; the plan's own choice
+        const int other = j.tiled.big_walk ? 0 : 1;
+        j.tiled.set_walk(other);
+        float ms2 = 0.0f;
+        if ((rc = time_it(ms2))) return rc;
+        best.walk = ms2 < std::min(ms, best_ms) ? other : -1;
     }
-    if ((rc = sloth_batch_set_tile(b, bw, bh))) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune[key] = std::make_pair(bw, bh);
+        g_tune[key] = best;
     }
-    if (tile_w) *tile_w = bw;
-    if (tile_h) *tile_h = bh;
+    return apply(best);
+}
+
+int sloth_batch_plan(const sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h, int* big_walk)
+{
+    ARG_CHECK(b, "null batch");
+    const Job& j = b->job;
+    const bool tiled = j.path == PATH_TILED;
+    if (tile_w) *tile_w = tiled ? j.tiled.TW : 0;
+    if (tile_h) *tile_h = tiled ? j.tiled.TH : 0;
+    if (big_walk) *big_walk = tiled ? (j.tiled.walk() ? 1 : 0) : -1;
     return SLOTH_OK;
 }
 

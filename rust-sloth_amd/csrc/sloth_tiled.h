@@ -65,6 +65,9 @@ struct TiledPlan {
     // one triangle in 512 or more) for the raster variant whose wavefronts
     // walk those boxes together; SLOTH_BIG_WALK=0/1 overrides
     bool big_walk = false;
+    // -1: the raster variant big_walk picks; 0 / 1: without / with the
+    // wavefront walk of big boxes (set by the autotuner)
+    int walk_force = -1;
 
     // Sets the tile grid of every frame and allocates per-tile and per-row
     // state.  Pool offsets become invalid until size() runs.
@@ -72,6 +75,13 @@ struct TiledPlan {
 
     // Tile size in cells: TW a power of two in 8..128, TH in 4..64.
     int set_tile(uint32_t tw, uint32_t th);
+    // The raster variant the next launches use (-1 = the sizing run's choice).
+    void set_walk(int mode)
+    {
+        if (mode != walk_force) ++generation;   // captured graphs hold the old variant
+        walk_force = mode;
+    }
+    bool walk() const { return walk_force >= 0 ? walk_force == 1 : big_walk; }
     size_t raster_lds() const;
 
     // Counting run over all frames (synchronous): per-tile list lengths ->

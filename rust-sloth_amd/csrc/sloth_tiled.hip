@@ -1110,8 +1110,8 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
         if (mx == 0 || my == 0) continue;
         bool one = true;   // every frame of the range has exactly one draw
         for (uint32_t f = f0; f < f1 && one; ++f) one = frames[f].draw_end - frames[f].draw_begin == 1;
-        const auto kern = big_walk ? (one ? k_tile_raster<true, true> : k_tile_raster<true, false>)
-                                   : (one ? k_tile_raster<false, true> : k_tile_raster<false, false>);
+        const auto kern = walk() ? (one ? k_tile_raster<true, true> : k_tile_raster<true, false>)
+                                 : (one ? k_tile_raster<false, true> : k_tile_raster<false, false>);
         hipLaunchKernelGGL(kern, dim3(mx, my, f1 - f0),
                            dim3(RASTER_BLOCK), raster_lds(), s, d_draws, d_ramps, d_frames, f0, log2u(TW), log2u(TH),
                            d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),

@@ -108,7 +108,9 @@ SIGNATURES = {
     "sloth_batch_profile": (C.c_int, [_vp, _fp, C.POINTER(C.c_char_p), _u32p]),
     "sloth_batch_time": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int, _fp]),
     "sloth_batch_set_tile": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    "sloth_batch_set_walk": (C.c_int, [_vp, C.c_int]),
     "sloth_batch_autotune": (C.c_int, [_vp, _u32p, _u32p]),
+    "sloth_batch_plan": (C.c_int, [_vp, _u32p, _u32p, _i32p]),
     "sloth_batch_destroy": (C.c_int, [_vp]),
     "sloth_shading_create": (C.c_int, [_vp, _u32p, C.c_uint32, C.c_uint32, _vpp]),
     "sloth_shading_visibility": (C.c_int, [_vp]),
@@ -540,11 +542,22 @@ class Batch:
     def set_tile(self, tile_w: int, tile_h: int) -> None:
         _check(load_library().sloth_batch_set_tile(self._h, tile_w, tile_h))
 
+    def set_walk(self, mode: int) -> None:
+        """Tile raster variant: 1 big boxes walked by wavefronts, 0 per lane,
+        -1 the sizing run's choice."""
+        _check(load_library().sloth_batch_set_walk(self._h, mode))
+
     def autotune(self) -> tuple:
         """Times the candidate tile sizes and keeps the fastest; returns it."""
         w, h = C.c_uint32(), C.c_uint32()
         _check(load_library().sloth_batch_autotune(self._h, C.byref(w), C.byref(h)))
         return int(w.value), int(h.value)
+
+    def plan(self) -> dict:
+        """The raster plan in use: tile size in cells and the big-box walk."""
+        w, h, bw = C.c_uint32(), C.c_uint32(), C.c_int32()
+        _check(load_library().sloth_batch_plan(self._h, C.byref(w), C.byref(h), C.byref(bw)))
+        return {"tile_w": int(w.value), "tile_h": int(h.value), "big_walk": int(bw.value)}
 
     def profile(self) -> dict:
         cap = 16

@@ -110,3 +110,30 @@ def test_uneven_batch_full(sloth, big_walk, monkeypatch):
         cells, z, _ = b.read_frame(i)
         ref = oracle.render(W, H, U, m.xyz, m.rgb, R)
         assert_frame_equal(cells, z, None, ref, W)
+
+
+def test_uneven_batch_autotuned(sloth):
+    """The uneven 64-mesh batch after the autotuner picked its tile size and
+    raster variant: every fourth mesh against the oracle, on two replays."""
+    from sloth_amd import meshgen
+
+    items = meshgen.workload_uneven_meshes()
+    frames, draws, keep = [], [], []
+    for i, (m, W, H, U, R) in enumerate(items):
+        sm = sloth.SimpleMesh.from_mesh(m)
+        keep.append(sm)
+        frames.append(sloth.FrameSpec(W, H, U, True))
+        draws.append(sloth.DrawSpec(i, sm, R))
+    b = sloth.Batch(frames, draws)
+    tw, th = b.autotune()
+    plan = b.plan()
+    say(f"case uneven batch autotuned: tile {tw}x{th}, big walk {plan['big_walk']}")
+    assert (plan["tile_w"], plan["tile_h"]) == (tw, th)
+    for rep in range(2):
+        b.run()
+        b.sync()
+        for i in range(0, 64, 4):
+            m, W, H, U, R = items[i]
+            cells, z, _ = b.read_frame(i)
+            ref = oracle.render(W, H, U, m.xyz, m.rgb, R)
+            assert_frame_equal(cells, z, None, ref, W)

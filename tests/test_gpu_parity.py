@@ -426,7 +426,8 @@ def test_autotune_keeps_results(sloth, raster_path):
     b = sloth.Batch([sloth.FrameSpec(W, H, U, True) for _ in Rs], [sloth.DrawSpec(k, sm, R) for k, R in enumerate(Rs)])
     tw, th = b.autotune()
     say(f"case autotune -> {tw}x{th}")
-    assert tw in (32, 64, 128) and th in (8, 16, 32)
+    assert tw in (16, 32, 64, 128) and th in (8, 16, 32, 64)
+    assert b.plan()["tile_w"] == tw and b.plan()["tile_h"] == th
     b.run()
     b.sync()
     for k, R in enumerate(Rs):
