@@ -234,6 +234,18 @@ int sloth_batch_read_frame(sloth_batch *b, uint32_t frame, uint32_t *cells, floa
 /* sloth_context_encode for one frame of a batch that has been run. */
 int sloth_batch_encode_frame(sloth_batch *b, uint32_t frame, int mode, char *out, uint64_t cap, uint64_t *len);
 
+/* The `-j` export (main.rs:55-58, 85-106) of frames [first, first + count)
+ * of a run batch: "let frames = [\n", then for each frame "`\n", the frame's
+ * flush bytes and "`,\n" ("`];\n" after the last).  mode is
+ * SLOTH_ENCODE_HTML (the colour export's spans, context.rs:79-87) or
+ * SLOTH_ENCODE_TEXT (with --no-color the export writes the plain flush,
+ * context.rs:58-61, newline included).  Which frames the export holds is the
+ * caller's schedule: main.rs stops after the requested count or once the
+ * accumulated pitch exceeds 9.42477.  With out = NULL only *len is set;
+ * otherwise out must hold cap >= *len bytes. */
+int sloth_batch_encode_export(sloth_batch *b, uint32_t first, uint32_t count, int mode, char *out, uint64_t cap,
+                              uint64_t *len);
+
 /* Device addresses of one frame's cells (W*H (+H) of them, starting on a
  * 256-byte boundary) and z (W*H). */
 int sloth_batch_device_frame(sloth_batch *b, uint32_t frame, uint32_t **d_cells, float **d_z);
@@ -348,6 +360,11 @@ int sloth_render_frame_noalloc(const float *d_positions, const uint8_t *d_colors
  * (context.rs:104-133), without a context. */
 int sloth_screen_transform(uint32_t width, uint32_t height, const float *bbox_max, uint32_t n_meshes,
                            float *out16);
+
+/* Rotation3::from_euler_angles(roll, pitch, yaw).to_homogeneous()
+ * (main.rs:76-77), column-major, with the C library's sinf / cosf as Rust's
+ * f32::sin_cos calls them. */
+int sloth_rotation_euler(float roll, float pitch, float yaw, float *out16);
 
 /* utransform * transform in nalgebra's Matrix4 product order
  * (rasterizer.rs:57). */

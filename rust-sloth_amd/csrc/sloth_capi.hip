@@ -607,6 +607,23 @@ int sloth_screen_transform(uint32_t width, uint32_t height, const float* bbox_ma
     return SLOTH_OK;
 }
 
+int sloth_rotation_euler(float roll, float pitch, float yaw, float* out16)
+{
+    ARG_CHECK(out16, "null output");
+    // Rotation3::from_euler_angles (main.rs:76-77) with the C library's
+    // binary32 sine and cosine, as Rust's f32::sin_cos; products in the
+    // order nalgebra writes them, then to_homogeneous
+    const float sr = sinf(roll), cr = cosf(roll);
+    const float sp = sinf(pitch), cp = cosf(pitch);
+    const float sy = sinf(yaw), cy = cosf(yaw);
+    const float m[16] = {cy * cp, sy * cp, -sp, 0.0f,
+                         cy * sp * sr - sy * cr, sy * sp * sr + cy * cr, cp * sr, 0.0f,
+                         cy * sp * cr + sy * sr, sy * sp * cr - cy * sr, cp * cr, 0.0f,
+                         0.0f, 0.0f, 0.0f, 1.0f};
+    memcpy(out16, m, sizeof m);
+    return SLOTH_OK;
+}
+
 int sloth_mat4_mul(const float* a16, const float* b16, float* out16)
 {
     ARG_CHECK(a16 && b16 && out16, "null argument");
@@ -1122,6 +1139,52 @@ int sloth_batch_encode_frame(sloth_batch* b, uint32_t frame, int mode, char* out
     const Frame& fr = b->job.frames[frame];
     const uint64_t nc = frame_cells(fr);
     return b->enc.run(b->job.d_cells.as<uint32_t>() + fr.cell_off, nc, fr.image != 0, mode, out, cap, len, b->stream);
+}
+
+int sloth_batch_encode_export(sloth_batch* b, uint32_t first, uint32_t count, int mode, char* out, uint64_t cap,
+                              uint64_t* len)
+{
+    ARG_CHECK(b && len, "null argument");
+    ARG_CHECK(count >= 1 && first < b->job.frames.size() && count <= b->job.frames.size() - first,
+              "frame range out of the batch");
+    ARG_CHECK(mode == SLOTH_ENCODE_HTML || mode == SLOTH_ENCODE_TEXT,
+              "the export encodes frames as SLOTH_ENCODE_HTML or SLOTH_ENCODE_TEXT");
+    if (!b->ran) return fail(SLOTH_ESTATE, "batch has not been run");
+    int rc = sloth_batch_sync(b);
+    if (rc) return rc;
+    // main.rs:55-58, 85-106: the header, then per frame a backtick line, the
+    // frame's flush and "`," -- "`];" after the last one
+    static const char HEAD[] = "let frames = [\n", OPEN[] = "`\n", NEXT[] = "`,\n", LAST[] = "`];\n";
+    uint64_t pos = 0;
+    auto put = [&](const char* txt, uint64_t n) -> int {
+        if (out) {
+            if (pos + n > cap)
+                return fail(SLOTH_EINVAL, "export buffer holds %llu bytes, more are needed", (unsigned long long)cap);
+            memcpy(out + pos, txt, n);
+        }
+        pos += n;
+        return SLOTH_OK;
+    };
+    if ((rc = put(HEAD, sizeof HEAD - 1))) return rc;
+    for (uint32_t f = first; f < first + count; ++f) {
+        if ((rc = put(OPEN, sizeof OPEN - 1))) return rc;
+        const Frame& fr = b->job.frames[f];
+        uint64_t n = 0;
+        if ((rc = b->enc.run(b->job.d_cells.as<uint32_t>() + fr.cell_off, frame_cells(fr), fr.image != 0, mode,
+                             nullptr, 0, &n, b->stream)))
+            return rc;
+        if (out) {
+            if (pos + n > cap)
+                return fail(SLOTH_EINVAL, "export buffer holds %llu bytes, more are needed", (unsigned long long)cap);
+            if ((rc = b->enc.run(b->job.d_cells.as<uint32_t>() + fr.cell_off, frame_cells(fr), fr.image != 0, mode,
+                                 out + pos, cap - pos, &n, b->stream)))
+                return rc;
+        }
+        pos += n;
+        if ((rc = f + 1 < first + count ? put(NEXT, sizeof NEXT - 1) : put(LAST, sizeof LAST - 1))) return rc;
+    }
+    *len = pos;
+    return SLOTH_OK;
 }
 
 int sloth_batch_device_frame(sloth_batch* b, uint32_t frame, uint32_t** d_cells, float** d_z)

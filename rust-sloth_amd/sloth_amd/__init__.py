@@ -105,6 +105,8 @@ SIGNATURES = {
     "sloth_batch_sync": (C.c_int, [_vp]),
     "sloth_batch_read_frame": (C.c_int, [_vp, C.c_uint32, _u32p, _fp, _i32p]),
     "sloth_batch_device_frame": (C.c_int, [_vp, C.c_uint32, _vpp, _vpp]),
+    "sloth_batch_encode_export": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_char_p, C.c_uint64,
+                                            C.POINTER(C.c_uint64)]),
     "sloth_batch_profile": (C.c_int, [_vp, _fp, C.POINTER(C.c_char_p), _u32p]),
     "sloth_batch_time": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int, _fp]),
     "sloth_batch_set_tile": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
@@ -125,6 +127,7 @@ SIGNATURES = {
                                              C.POINTER(sloth_ramp), _vp, _vp, _vp]),
     "sloth_screen_transform": (C.c_int, [C.c_uint32, C.c_uint32, _fp, C.c_uint32, _fp]),
     "sloth_mat4_mul": (C.c_int, [_fp, _fp, _fp]),
+    "sloth_rotation_euler": (C.c_int, [C.c_float, C.c_float, C.c_float, _fp]),
 }
 
 _lib = None
@@ -440,6 +443,15 @@ def screen_transform(width: int, height: int, bbox_maxes) -> np.ndarray:
     return out
 
 
+def rotation_euler(roll: float, pitch: float, yaw: float) -> np.ndarray:
+    """Rotation3::from_euler_angles(...).to_homogeneous() (main.rs:76-77),
+    column-major float32[16], from the library's host helper."""
+    out = np.empty(16, np.float32)
+    _check(load_library().sloth_rotation_euler(float(np.float32(roll)), float(np.float32(pitch)),
+                                               float(np.float32(yaw)), _fptr(out)))
+    return out
+
+
 def mat4_mul(a, b) -> np.ndarray:
     a = np.ascontiguousarray(a, np.float32).reshape(16)
     b = np.ascontiguousarray(b, np.float32).reshape(16)
@@ -525,6 +537,12 @@ class Batch:
 
     def encode_frame(self, f: int, mode: int = ENCODE_TEXT) -> bytes:
         return _encode(load_library().sloth_batch_encode_frame, self._h, f, mode)
+
+    def encode_export(self, first: int = 0, count: int | None = None, mode: int = ENCODE_HTML) -> bytes:
+        """The `-j` export of frames [first, first + count) (main.rs:55-58,
+        85-106), encoded on the GPU."""
+        count = len(self.frames) - first if count is None else count
+        return _encode(load_library().sloth_batch_encode_export, self._h, first, count, mode)
 
     def device_frame(self, f: int):
         c = C.c_void_p()

@@ -266,6 +266,22 @@ def turntable_pitches(n_frames: int, start_pitch: float = 0.0) -> np.ndarray:
     return out
 
 
+def export_pitches(n_frames: int, start_pitch: float = 0.0) -> np.ndarray:
+    """The pitches the `-j n_frames` export renders (main.rs:55-58, 88-106):
+    the turntable schedule, stopped after n_frames frames or as soon as the
+    pitch after a frame exceeds 9.42477 (then that frame is the last)."""
+    pi = _f32(math.pi)
+    pitch = _f32(_f32(start_pitch) + pi)
+    step = _f32(_f32(_f32(2.0) * pi) * _f32(_f32(1.0) / _f32(n_frames)))
+    out = []
+    for k in range(n_frames):
+        out.append(pitch)
+        pitch = _f32(pitch + step)
+        if pitch > _f32(9.42477) or k == n_frames - 1:
+            break
+    return np.array(out, np.float32)
+
+
 def screen_transform(width: int, height: int, bbox_maxes) -> np.ndarray:
     """Context::update (context.rs:93-141): utransform from the size and the
     largest bounding-box maximum over the meshes, column-major float32[16]."""

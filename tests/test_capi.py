@@ -89,3 +89,35 @@ def test_default_shader_table():
     assert r.n == 9 and bytes(r.glyphs[:9]) == b".:-=+*#%@" and r.fallback == ord(" ")
     assert [np.float32(r.thresholds[i]) for i in range(9)] == [
         np.float32(x) for x in (0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0)]
+
+
+def test_rotation_helper_matches_oracle():
+    """sloth_rotation_euler (host code of the library, C library sinf/cosf)
+    is bit for bit the oracle's Rotation3::from_euler_angles."""
+    import numpy as np
+
+    import oracle
+    import sloth_amd
+    from sloth_amd import meshgen
+
+    rng = np.random.default_rng(5)
+    cases = [(0.0, float(p), 0.0) for p in meshgen.turntable_pitches(256)]
+    cases += [tuple(float(v) for v in r) for r in rng.uniform(-7, 7, (2000, 3)).astype(np.float32)]
+    for r, p, y in cases:
+        a = sloth_amd.rotation_euler(r, p, y)
+        b = oracle.rotation_euler(r, p, y)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (r, p, y)
+
+
+def test_export_schedule():
+    """The -j export stops after the requested count or once the pitch after
+    a frame exceeds 9.42477 (main.rs:88-106)."""
+    import numpy as np
+
+    from sloth_amd import meshgen
+
+    p = meshgen.export_pitches(100)
+    assert len(p) == 100 and np.array_equal(p, meshgen.turntable_pitches(100))
+    q = meshgen.export_pitches(20, 2.5)
+    step = np.float32(np.float32(np.float32(2.0) * np.float32(np.pi)) * np.float32(np.float32(1.0) / np.float32(20)))
+    assert len(q) < 20 and np.float32(q[-1] + step) > np.float32(9.42477) and q[-1] <= np.float32(9.42477)

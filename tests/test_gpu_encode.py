@@ -87,3 +87,41 @@ def test_encode_terminal_mode_and_odd_cell_counts(sloth):
             got = ctx.encode(mode)
             assert got == oracle.encode(cells, mode, image=image)
             assert got.startswith(b"\x1b[1;1H") == (not image)
+
+
+@pytest.mark.parametrize("n_frames,start,mode", [(20, 0.0, 2), (20, 2.5, 2), (7, 0.4, 0)])
+def test_export_framing(sloth, n_frames, start, mode):
+    """The `-j N` export of the reference's skull.obj at 100x100 (main.rs:55-58,
+    85-106): the frames of the export schedule in one batch, encoded with the
+    framing around each frame's flush, against the oracle's frames framed the
+    same way.  A start pitch of 2.5 ends the export early (pitch > 9.42477);
+    mode 0 is the --no-color export (plain flush, newline included)."""
+    import os
+
+    from sloth_amd import meshgen
+
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "skull_datajs.npz"))
+    xyz, rgb, counts = d["xyz"], d["rgb"], d["mesh_counts"]
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    W, H = 100, 100
+    U = meshgen.screen_transform(W, H, [oracle.mesh_bbox(xyz[offs[i]:offs[i + 1]])[1] for i in range(len(counts))])
+    meshes = [sloth.SimpleMesh(xyz[offs[i]:offs[i + 1]], rgb[offs[i]:offs[i + 1]]) for i in range(len(counts))]
+    pitches = meshgen.export_pitches(n_frames, start)
+    frames = [sloth.FrameSpec(W, H, U, True) for _ in pitches]
+    draws = [sloth.DrawSpec(f, m, sloth.rotation_euler(0.0, float(p), 0.0))
+             for f, p in enumerate(pitches) for m in meshes]
+    b = sloth.Batch(frames, draws)
+    b.run()
+    b.sync()
+    say(f"case export framing: {len(pitches)} of {n_frames} frames, start {start}, mode {mode}")
+    got = b.encode_export(mode=mode)
+    want = [b"let frames = [\n"]
+    for k, p in enumerate(pitches):
+        f = oracle.Frame(W, H, True)
+        R = oracle.rotation_euler(0.0, float(p), 0.0)
+        for i in range(len(counts)):
+            f.draw_mesh(U, xyz[offs[i]:offs[i + 1]], rgb[offs[i]:offs[i + 1]], R)
+        want += [b"`\n", oracle.encode(f.cells, mode, image=True), b"`,\n" if k + 1 < len(pitches) else b"`];\n"]
+    assert got == b"".join(want)
+    if start == 2.5:
+        assert len(pitches) < n_frames
