@@ -86,6 +86,20 @@ int sloth_mesh_create(const float *positions, const uint8_t *colors, uint32_t n,
 int sloth_mesh_create_device(const float *d_positions, const uint8_t *d_colors, uint32_t n,
                              sloth_mesh **out);
 
+/* An indexed device mesh: vertex rows float[n_vertices][3] and per triangle
+ * its three vertex ids faces uint32[n][3] and its colour uint8[n][3].  It
+ * renders exactly as the soup positions[t][k] = vertices[faces[t][k]] (the
+ * kernels read the same floats through the ids), in 12 bytes per vertex and
+ * 15 per triangle instead of 39 per triangle.  Face ids must be below
+ * n_vertices (else SLOTH_EINVAL).  The bounding box follows the OBJ loader
+ * over the triangles.  No reference counterpart: SimpleMesh is a soup
+ * (geometry.rs:78-81). */
+int sloth_mesh_create_indexed(const float *vertices, uint32_t n_vertices, const uint32_t *faces,
+                              const uint8_t *colors, uint32_t n, sloth_mesh **out);
+
+/* 1 if the mesh is indexed (and its vertex count in *n_vertices), else 0. */
+int sloth_mesh_is_indexed(const sloth_mesh *mesh, uint32_t *n_vertices);
+
 /* SimpleMesh.bounding_box (geometry.rs:79): min[3], max[3]. */
 int sloth_mesh_bbox(const sloth_mesh *mesh, float *min3, float *max3);
 uint32_t sloth_mesh_triangles(const sloth_mesh *mesh);
@@ -110,8 +124,9 @@ int sloth_model_mesh_count(const sloth_model *model, uint32_t *n_meshes);
  * is destroyed), triangle count and bounding box; any output may be NULL. */
 int sloth_model_mesh(const sloth_model *model, uint32_t i, const float **positions, const uint8_t **colors,
                      uint32_t *n_tris, float *bbox_min3, float *bbox_max3);
-/* Mesh i uploaded as a device mesh; sloth_mesh_bbox then reports the
- * loader's bounding box. */
+/* Mesh i uploaded as a device mesh, indexed: positions the soup repeats bit
+ * for bit are stored once (it renders as the soup does); sloth_mesh_bbox
+ * then reports the loader's bounding box. */
 int sloth_model_upload(const sloth_model *model, uint32_t i, sloth_mesh **out);
 int sloth_model_destroy(sloth_model *model);
 

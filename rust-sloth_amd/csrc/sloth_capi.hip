@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <unordered_map>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -141,10 +143,11 @@ static bool same_ramp(const Ramp& a, const Ramp& b)
 // meshes
 // ---------------------------------------------------------------------------
 struct sloth_mesh {
-    const float* d_xyz = nullptr;
+    const float* d_xyz = nullptr;       // soup [n][3][3], or vertex rows [n_verts][3] when d_faces is set
+    const uint32_t* d_faces = nullptr;  // indexed meshes: [n][3] vertex ids
     const uint8_t* d_rgb = nullptr;
-    uint32_t n = 0;
-    DevBuf own_xyz, own_rgb;
+    uint32_t n = 0, n_verts = 0;
+    DevBuf own_xyz, own_rgb, own_faces;
     float bbmin[3] = {0, 0, 0}, bbmax[3] = {0, 0, 0};
 };
 
@@ -232,6 +235,7 @@ struct Job {
                 const DrawIn& di = din[order[k]];
                 Draw& d = draws[k];
                 d.xyz = di.mesh->d_xyz;
+                d.faces = di.mesh->d_faces;
                 d.rgb = di.mesh->d_rgb;
                 d.n = di.mesh->n;
                 d.tri_begin = tri;
@@ -693,6 +697,60 @@ int sloth_mesh_create_device(const float* d_positions, const uint8_t* d_colors, 
     return SLOTH_OK;
 }
 
+int sloth_mesh_create_indexed(const float* vertices, uint32_t n_vertices, const uint32_t* faces,
+                              const uint8_t* colors, uint32_t n, sloth_mesh** out)
+{
+    ARG_CHECK(out, "null output");
+    *out = nullptr;
+    ARG_CHECK(n == 0 || (vertices && faces && colors), "null vertices, faces or colours");
+    for (uint64_t i = 0; i < (uint64_t)n * 3; ++i)
+        if (faces[i] >= n_vertices) return fail(SLOTH_EINVAL, "face %llu names vertex %u of %u", (unsigned long long)(i / 3),
+                                                faces[i], n_vertices);
+    int rc = ensure_device();
+    if (rc) return rc;
+    sloth_mesh* m = new (std::nothrow) sloth_mesh();
+    if (!m) return fail(SLOTH_ENOMEM, "host allocation failed");
+    m->n = n;
+    m->n_verts = n_vertices;
+    // the OBJ bounding box (geometry.rs:85-88, 129-135) over the triangles
+    for (int c = 0; c < 3; ++c) m->bbmin[c] = m->bbmax[c] = 0.0f;
+    for (uint32_t t = 0; t < n; ++t)
+        for (int c = 0; c < 3; ++c) {
+            const float a = vertices[3 * (size_t)faces[3 * (size_t)t] + c];
+            const float b = vertices[3 * (size_t)faces[3 * (size_t)t + 1] + c];
+            const float d = vertices[3 * (size_t)faces[3 * (size_t)t + 2] + c];
+            m->bbmin[c] = fminf(fminf(a, fminf(b, d)), m->bbmin[c]);
+            m->bbmax[c] = fmaxf(fmaxf(a, fmaxf(b, d)), m->bbmax[c]);
+        }
+    if (n) {
+        const size_t vb = (size_t)n_vertices * 12, fb = (size_t)n * 12, cb = (size_t)n * 3;
+        if ((rc = m->own_xyz.reserve(std::max<size_t>(vb, 4))) || (rc = m->own_faces.reserve(fb)) ||
+            (rc = m->own_rgb.reserve(cb))) {
+            delete m;
+            return rc;
+        }
+        hipError_t e1 = vb ? hipMemcpy(m->own_xyz.p, vertices, vb, hipMemcpyHostToDevice) : hipSuccess;
+        hipError_t e2 = hipMemcpy(m->own_faces.p, faces, fb, hipMemcpyHostToDevice);
+        hipError_t e3 = hipMemcpy(m->own_rgb.p, colors, cb, hipMemcpyHostToDevice);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+            delete m;
+            return fail(SLOTH_EHIP, "indexed mesh upload failed");
+        }
+        m->d_xyz = m->own_xyz.as<float>();
+        m->d_faces = m->own_faces.as<uint32_t>();
+        m->d_rgb = m->own_rgb.as<uint8_t>();
+    }
+    *out = m;
+    return SLOTH_OK;
+}
+
+int sloth_mesh_is_indexed(const sloth_mesh* mesh, uint32_t* n_vertices)
+{
+    ARG_CHECK(mesh, "null mesh");
+    if (n_vertices) *n_vertices = mesh->d_faces ? mesh->n_verts : 0;
+    return mesh->d_faces ? 1 : 0;
+}
+
 int sloth_mesh_bbox(const sloth_mesh* mesh, float* min3, float* max3)
 {
     ARG_CHECK(mesh, "null mesh");
@@ -709,6 +767,16 @@ uint32_t sloth_mesh_triangles(const sloth_mesh* mesh) { return mesh ? mesh->n : 
 
 struct sloth_model {
     std::vector<SoupMesh> meshes;
+};
+
+// Hash of a vertex's three binary32 bit patterns.
+struct VertexHash {
+    size_t operator()(const std::array<uint32_t, 3>& v) const
+    {
+        uint64_t h = 1469598103934665603ull;
+        for (uint32_t x : v) h = (h ^ x) * 1099511628211ull;
+        return (size_t)h;
+    }
 };
 
 extern "C" {
@@ -760,7 +828,24 @@ int sloth_model_upload(const sloth_model* model, uint32_t i, sloth_mesh** out)
     ARG_CHECK(model && out, "null argument");
     ARG_CHECK(i < model->meshes.size(), "mesh index out of range");
     const SoupMesh& m = model->meshes[i];
-    int rc = sloth_mesh_create(m.xyz.data(), m.rgb.data(), m.n(), out);
+    // Vertices that the soup repeats bit for bit are stored once: an indexed
+    // device mesh of the same floats (renders identically to the soup)
+    std::vector<float> verts;
+    std::vector<uint32_t> faces;
+    try {
+        std::unordered_map<std::array<uint32_t, 3>, uint32_t, VertexHash> ids;
+        faces.resize((size_t)m.n() * 3);
+        for (size_t k = 0; k < (size_t)m.n() * 3; ++k) {
+            std::array<uint32_t, 3> key;
+            memcpy(key.data(), &m.xyz[k * 3], 12);
+            auto it = ids.emplace(key, (uint32_t)(verts.size() / 3));
+            if (it.second) verts.insert(verts.end(), &m.xyz[k * 3], &m.xyz[k * 3] + 3);
+            faces[k] = it.first->second;
+        }
+    } catch (const std::bad_alloc&) {
+        return fail(SLOTH_ENOMEM, "host allocation failed");
+    }
+    int rc = sloth_mesh_create_indexed(verts.data(), (uint32_t)(verts.size() / 3), faces.data(), m.rgb.data(), m.n(), out);
     if (rc) return rc;
     // the loader's bounding-box rule (an STL box does not start at the origin)
     memcpy((*out)->bbmin, m.bbmin, 12);

@@ -28,7 +28,8 @@ struct Ramp {
 
 // One draw_mesh call (rasterizer.rs:39-46) inside one frame.
 struct Draw {
-    const float* xyz;     // [n][3][3] positions, w = 1
+    const float* xyz;     // [n][3][3] positions, w = 1 (a soup); or [V][3] vertex rows when faces is set
+    const uint32_t* faces;   // [n][3] vertex ids of an indexed mesh, or null for a soup
     const uint8_t* rgb;   // [n][3]
     uint32_t n;           // triangles
     uint32_t tri_begin;   // frame-global index of the first triangle (tie order)
@@ -100,9 +101,10 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* global_ptr
     return (const __attribute__((address_space(1))) T*)p;
 }
 
-__device__ __forceinline__ void setup_tri(const float* M, const float* pv, uint32_t W, uint32_t H, Tri& t)
+// Transformed triangle from its 9 model-space coordinates p (v1, v2, v3).
+template <class P>
+__device__ __forceinline__ void setup_tri_from(const float* M, const P& p, uint32_t W, uint32_t H, Tri& t)
 {
-    const auto* p = global_ptr(pv);
     t.x1 = xform_row(M, 0, p[0], p[1], p[2]);
     t.y1 = xform_row(M, 1, p[0], p[1], p[2]);
     t.z1 = xform_row(M, 2, p[0], p[1], p[2]);
@@ -128,6 +130,26 @@ __device__ __forceinline__ void setup_tri(const float* M, const float* pv, uint3
     t.B2 = t.y2 - t.y1;
     float o = t.A2 * (t.y3 - t.y1) - t.B2 * (t.x3 - t.x1);
     t.a = 1.0f / o;
+}
+
+__device__ __forceinline__ void setup_tri(const float* M, const float* pv, uint32_t W, uint32_t H, Tri& t)
+{
+    setup_tri_from(M, global_ptr(pv), W, H, t);
+}
+
+// Triangle `local` of a draw: a soup's 9 consecutive floats, or the vertex
+// rows an indexed mesh's face names (the same floats, so the same results).
+__device__ __forceinline__ void setup_draw_tri(const Draw& dr, uint32_t local, uint32_t W, uint32_t H, Tri& t)
+{
+    if (dr.faces) {
+        const auto* f = global_ptr(dr.faces + (size_t)local * 3);
+        const auto* vp = global_ptr(dr.xyz);
+        const uint32_t a = 3u * f[0], b = 3u * f[1], c = 3u * f[2];
+        const float v[9] = {vp[a], vp[a + 1], vp[a + 2], vp[b], vp[b + 1], vp[b + 2], vp[c], vp[c + 1], vp[c + 2]};
+        setup_tri_from(dr.M, v, W, H, t);
+    } else {
+        setup_tri(dr.M, dr.xyz + (size_t)local * 9, W, H, t);
+    }
 }
 
 // The three edge functions at integer pixel (x, y) (rasterizer.rs:71-74).

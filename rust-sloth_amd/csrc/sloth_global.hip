@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(256) k_raster_global(const Draw* __restrict__ 
     const Frame& fr = frames[dr.frame];
     const uint32_t W = fr.W, H = fr.H;
     Tri t;
-    setup_tri(dr.M, dr.xyz + (size_t)local * 9, W, H, t);
+    setup_draw_tri(dr, local, W, H, t);
     const uint32_t tri = dr.tri_begin + local;
     uint64_t* fk = keys + fr.z_off;
 
@@ -137,7 +137,7 @@ __device__ __forceinline__ void winner_fragment(const Draw* __restrict__ draws, 
     auto derive = [&](const Draw& dr) {
         uint32_t local = tri - dr.tri_begin;
         Tri t;
-        setup_tri(dr.M, dr.xyz + (size_t)local * 9, fr.W, fr.H, t);
+        setup_draw_tri(dr, local, fr.W, fr.H, t);
         float w0, w1, w2;
         edges(t, (float)px, (float)py, w0, w1, w2);
         z = depth(t, w1, w2);

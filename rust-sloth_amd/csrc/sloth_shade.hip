@@ -55,7 +55,7 @@ __device__ __forceinline__ Fragment shade_fragment(const ShadeArgs& p, const Fra
     const Draw& dr = p.draws[d];
     const uint32_t li = T - dr.tri_begin;
     Tri t;
-    setup_tri(dr.M, dr.xyz + (size_t)li * 9, fr.W, fr.H, t);
+    setup_draw_tri(dr, li, fr.W, fr.H, t);
     float w0, w1, w2;
     edges(t, (float)px, (float)py, w0, w1, w2);
     f.lam[0] = t.a * w0;

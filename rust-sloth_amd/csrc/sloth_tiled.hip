@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     // that the set-up below has no branch around it (the lanes of a wavefront
     // take different classes; selects cost less than divergent paths).
     Tri t;
-    setup_tri(dr.M, dr.xyz + (size_t)(valid ? local : dr.n - 1) * 9, W, H, t);
+    setup_draw_tri(dr, valid ? local : dr.n - 1, W, H, t);
     const uint32_t cls = valid ? setup_class(t) : CLS_EMPTY;
     const bool boxed = cls == CLS_BOX || cls == CLS_TIGHT;
     const bool careful = cls == CLS_SPANS || cls == CLS_FULL;
@@ -385,7 +385,7 @@ __device__ __forceinline__ void careful_one(const Draw* __restrict__ draws, cons
     const uint32_t frame = dr.frame;
     const Frame& fr = frames[frame];
     Tri t;
-    setup_tri(dr.M, dr.xyz + (size_t)in.y * 9, fr.W, fr.H, t);
+    setup_draw_tri(dr, in.y, fr.W, fr.H, t);
     const uint32_t T = dr.tri_begin + in.y;
     const uint32_t cls = setup_class(t);
     int bx0 = t.x0, bx1 = t.x1d - 1, by0 = t.y0, by1 = t.y1d - 1;
@@ -775,7 +775,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 const Draw& dr = ONE ? draws[fr.draw_begin] : one_draw ? draws[fr.draw_begin] : draw_of_t(draws, fr, T);
                 const uint32_t li = T - dr.tri_begin;
                 Tri t;
-                setup_tri(dr.M, dr.xyz + (size_t)li * 9, W, H, t);
+                setup_draw_tri(dr, li, W, H, t);
                 const float na = unit_normal_z(t) * t.a;   // rasterizer.rs:76 multiplies these first
                 const auto* c = global_ptr(dr.rgb + (size_t)li * 3);
                 const uint32_t rgb = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24);

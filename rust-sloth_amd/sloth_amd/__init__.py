@@ -73,6 +73,8 @@ SIGNATURES = {
     "sloth_default_ramp": (C.c_int, [C.POINTER(sloth_ramp)]),
     "sloth_mesh_create": (C.c_int, [_fp, _u8p, C.c_uint32, _vpp]),
     "sloth_mesh_create_device": (C.c_int, [_vp, _vp, C.c_uint32, _vpp]),
+    "sloth_mesh_create_indexed": (C.c_int, [_fp, C.c_uint32, _u32p, _u8p, C.c_uint32, _vpp]),
+    "sloth_mesh_is_indexed": (C.c_int, [_vp, _u32p]),
     "sloth_mesh_bbox": (C.c_int, [_vp, _fp, _fp]),
     "sloth_mesh_triangles": (C.c_uint32, [_vp]),
     "sloth_mesh_destroy": (C.c_int, [_vp]),
@@ -246,8 +248,41 @@ class SimpleMesh:
         self.bounding_box = AABB(mn, mx)
 
     @classmethod
-    def from_mesh(cls, m) -> "SimpleMesh":
+    def from_mesh(cls, m, indexed: bool = False) -> "SimpleMesh":
+        """A device copy of a meshgen Mesh: its soup, or (indexed=True) its
+        vertex rows and faces."""
+        if indexed:
+            return cls.indexed(m.verts, m.faces, m.rgb)
         return cls(m.xyz, m.rgb)
+
+    @classmethod
+    def indexed(cls, vertices, faces, colors) -> "SimpleMesh":
+        """An indexed device mesh (sloth_mesh_create_indexed): vertex rows
+        [V,3] float32, faces [n,3] vertex ids, colours [n,3] uint8; renders as
+        the soup vertices[faces]."""
+        _ensure_init()
+        L = load_library()
+        self = cls.__new__(cls)
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        c = np.ascontiguousarray(colors, np.uint8).reshape(-1, 3)
+        if f.shape[0] != c.shape[0]:
+            raise ValueError("faces and colors disagree on the triangle count")
+        h = C.c_void_p()
+        _check(L.sloth_mesh_create_indexed(_fptr(v), v.shape[0], f.ctypes.data_as(_u32p), c.ctypes.data_as(_u8p),
+                                           f.shape[0], C.byref(h)))
+        self._h = h
+        self.positions = v[f.astype(np.int64)] if f.size else np.zeros((0, 3, 3), np.float32)
+        self.colors = c
+        mn = np.zeros(3, np.float32)
+        mx = np.zeros(3, np.float32)
+        _check(L.sloth_mesh_bbox(h, _fptr(mn), _fptr(mx)))
+        self.bounding_box = AABB(mn, mx)
+        return self
+
+    @property
+    def is_indexed(self) -> bool:
+        return bool(load_library().sloth_mesh_is_indexed(self._h, None))
 
     @classmethod
     def _from_model(cls, model, i: int) -> "SimpleMesh":

@@ -485,3 +485,46 @@ def test_near_threshold_shades(sloth):
     I = np.eye(4, dtype=np.float32).reshape(16)
     rgb = rng.integers(0, 256, (len(t), 3), dtype=np.uint8)
     _check(sloth, "near-threshold shades", W, H, I, [(t, rgb, I, None)])
+
+
+def test_indexed_meshes(sloth):
+    """Indexed device meshes (vertex rows + face ids) render as the soups
+    vertices[faces]: a frame of two indexed draws and one soup draw against
+    the oracle on the soups, and a batch of indexed draws against the same
+    batch of soups, cell for cell."""
+    from sloth_amd import meshgen
+
+    ico = meshgen.displaced_icosphere(3, seed=31)
+    box = meshgen.subdivided_box(4, seed=32)
+    sph = meshgen.uv_sphere(20, 10, seed=33)
+    assert sloth.SimpleMesh.from_mesh(ico, indexed=True).is_indexed
+    W, H = 180, 70
+    U = meshgen.screen_transform(W, H, [m.bbox_max() for m in (ico, box, sph)])
+    Rs = [meshgen.rotation_euler(0.3, 1.3, 0.2), meshgen.rotation_euler(1.0, 2.0, 0.5),
+          meshgen.rotation_euler(0.0, 0.4, 0.0)]
+    say(f"case indexed meshes {W}x{H}")
+    ctx = sloth.Context(W, H, image=True)
+    ctx.utransform = U
+    ctx.clear()
+    gm = [sloth.SimpleMesh.from_mesh(ico, indexed=True), sloth.SimpleMesh.from_mesh(box),
+          sloth.SimpleMesh.from_mesh(sph, indexed=True)]
+    for m, R in zip(gm, Rs):
+        sloth.draw_mesh(ctx, m, R)
+    cells, z, win = ctx.read(want_winners=True)
+    ref = oracle_render(W, H, U, [(m.xyz, m.rgb, R, None) for m, R in zip((ico, box, sph), Rs)])
+    assert_frame_equal(cells, z, win, ref, W)
+
+    frames = [sloth.FrameSpec(W, H, U, True) for _ in range(4)]
+    for indexed in (True, False):
+        mesh = sloth.SimpleMesh.from_mesh(ico, indexed=indexed)
+        b = sloth.Batch(frames, [sloth.DrawSpec(f, mesh, meshgen.rotation_euler(0.1 * f, 0.7 * f, 0.0))
+                                 for f in range(4)])
+        b.run()
+        b.sync()
+        out = [b.read_frame(f, want_winners=True) for f in range(4)]
+        if indexed:
+            first = out
+        else:
+            for (c0, z0, w0), (c1, z1, w1) in zip(first, out):
+                assert np.array_equal(c0, c1) and np.array_equal(z0.view(np.uint32), z1.view(np.uint32))
+                assert np.array_equal(w0, w1)

@@ -184,3 +184,35 @@ def test_reduces_to_flat_shade(sloth):
         assert abs(float(out[idx, 0]) - float(shade)) <= 1e-6, (idx, out[idx], shade)
         checked += 1
     assert checked > 50
+
+
+def test_shading_on_indexed_mesh(sloth):
+    """The shading pass reads the batch's triangles through an indexed mesh
+    as through a soup: the same outputs and loss bit for bit, and the same
+    gradients up to the order of their atomic sums."""
+    from sloth_amd import meshgen
+
+    m = meshgen.uv_sphere(24, 12, 1.0, seed=21)
+    rng = np.random.default_rng(21)
+    W, H, V = 96, 64, 2
+    U = meshgen.screen_transform(W, H, [m.bbox_max()])
+    Rs = [meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3)) for _ in range(V)]
+    vc = rng.uniform(0, 1, (m.verts.shape[0], 3)).astype(np.float32)
+    vn = (m.verts / np.linalg.norm(m.verts, axis=1, keepdims=True)).astype(np.float32)
+    L = np.array([0.577, 0.577, -0.577], np.float32)
+    say("case shading on an indexed mesh")
+    res = []
+    for indexed in (False, True):
+        mesh = sloth.SimpleMesh.from_mesh(m, indexed=indexed)
+        b = sloth.Batch([sloth.FrameSpec(W, H, U, True) for _ in range(V)],
+                        [sloth.DrawSpec(f, mesh, Rs[f]) for f in range(V)])
+        sh = sloth.Shading(b, m.faces, m.verts.shape[0])
+        _, out = sh.forward(vc, vn, L, want_out=True)
+        sh.set_target(np.zeros_like(out))
+        res.append((out, sh.step(vc, vn, L)))
+    (o0, s0), (o1, s1) = res
+    assert np.array_equal(o0, o1) and s0[0] == s1[0]
+    # the gradients are binary64 atomic sums, whose order varies from run to
+    # run: equal to the last few binary32 bits
+    for a, b in zip(s0[1:], s1[1:]):
+        assert np.linalg.norm(a.astype(np.float64) - b) <= 1e-6 * np.linalg.norm(a.astype(np.float64))
