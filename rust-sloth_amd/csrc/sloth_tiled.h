@@ -51,6 +51,11 @@ constexpr uint32_t PIPE_MIN_FRAMES = 16;  // frames per group at least
 
 struct TiledPlan {
     uint32_t TW = 64, TH = 32;       // tile size in cells (TW even)
+    bool user_tile = false;          // set_tile was called (keeps the size on a re-plan)
+    // every frame has an odd width: a z slot on every cell column, cells
+    // resolved from two slots, first-column cells finished by k_tile_edges
+    bool odd = false;
+    DevBuf d_edges;                  // odd: per tile and row, the first and last column slots' winners
     uint64_t n_tiles = 0;
     uint64_t n_rows = 0;
     bool sized = false;              // pool offsets match the current layout
@@ -70,8 +75,9 @@ struct TiledPlan {
     int walk_force = -1;
 
     // Sets the tile grid of every frame and allocates per-tile and per-row
-    // state.  Pool offsets become invalid until size() runs.
-    int plan(std::vector<Frame>& frames, uint64_t rows);
+    // state.  Pool offsets become invalid until size() runs.  odd_frames:
+    // every frame has an odd width.
+    int plan(std::vector<Frame>& frames, uint64_t rows, bool odd_frames);
 
     // Tile size in cells: TW a power of two in 8..128, TH in 4..64.
     int set_tile(uint32_t tw, uint32_t th);
@@ -81,7 +87,7 @@ struct TiledPlan {
         if (mode != walk_force) ++generation;   // captured graphs hold the old variant
         walk_force = mode;
     }
-    bool walk() const { return walk_force >= 0 ? walk_force == 1 : big_walk; }
+    bool walk() const { return !odd && (walk_force >= 0 ? walk_force == 1 : big_walk); }
     size_t raster_lds() const;
 
     // Counting run over all frames (synchronous): per-tile list lengths ->

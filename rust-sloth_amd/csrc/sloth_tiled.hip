@@ -63,16 +63,16 @@ __device__ __forceinline__ void empty_rects(TileRects& r)
 __device__ __forceinline__ void tile_rects(int bx0, int bx1, int by0, int by1, uint32_t W, int tws, int ths,
                                            TileRects& r)
 {
-    const int half = (int)(W >> 1);
+    // direct pixels: 2x < W, i.e. x <= (W - 1) / 2; wrapped: x >= (W + 1) / 2
     empty_rects(r);
-    int dxe = min(bx1, half - 1);
+    int dxe = min(bx1, (int)((W - 1) >> 1));
     if (bx0 <= dxe && by0 <= by1) {
         r.dx0 = (2 * bx0) >> tws;
         r.dx1 = (2 * dxe) >> tws;
         r.dy0 = by0 >> ths;
         r.dy1 = by1 >> ths;
     }
-    int wxs = max(bx0, half);
+    int wxs = max(bx0, (int)((W + 1) >> 1));
     if (wxs <= bx1 && by0 <= by1) {
         r.wx0 = (2 * wxs - (int)W) >> tws;
         r.wx1 = (2 * bx1 - (int)W) >> tws;
@@ -490,33 +490,41 @@ __device__ __forceinline__ const Draw& draw_of_t(const Draw* __restrict__ draws,
     return draws[lo];
 }
 
+// A tile's cells and z slots.  With W even a z slot id = y*W + 2x sits on an
+// even column and owns its two cells: a tile row of TW cells holds TW/2
+// slots (cs = 1).  With W odd, direct pixels land on even columns and
+// wrapped ones (2x >= W, row y + 1) on odd columns, so every column holds a
+// slot (cs = 0).  A pixel whose slot is cell column c of the tile's row r
+// takes slot ((r - tr0) << sws) + ((c - tc0) >> cs).
 struct TileGeom {
     int tc0, tc1, tr0, tr1;   // cell columns [tc0,tc1), rows [tr0,tr1)
-    int sws;                  // log2(slots per tile row) = log2(TW/2)
+    int sws;                  // log2(slots per tile row): log2(TW >> cs)
+    int cs;                   // 1 for even W, 0 for odd W
     int W;
 };
 
 // Calls f(px, py, slot, sub) for the pixels (x, y) in [xa, xb] x [ya, yb],
-// px = x and py = y as floats, slot = row0 + ((y - ya) << sws) + (x - xa).
+// px = x and py = y as floats, slot = row0 + ((y - ya) << sws) + (x - xa) * xs.
 // Pixel coordinates are below 2^16, so stepping px by 1.0f is exact (one add
 // instead of a conversion per pixel).
 template <class Fn>
-__device__ __forceinline__ void walk_rect(int xa, int xb, int ya, int yb, int row0, int sws, uint32_t sub, Fn&& f)
+__device__ __forceinline__ void walk_rect(int xa, int xb, int ya, int yb, int row0, int sws, int xs, uint32_t sub,
+                                          Fn&& f)
 {
     const float fxb = (float)xb;
     for (int y = ya; y <= yb; ++y, row0 += 1 << sws) {
         const float py = (float)y;
         int slot = row0;
-        for (float px = (float)xa; px <= fxb; px += 1.0f, ++slot) f(px, py, slot, sub);
+        for (float px = (float)xa; px <= fxb; px += 1.0f, slot += xs) f(px, py, slot, sub);
     }
 }
 
 // walk_rect over the direct region with each row cut to its candidate span
 // (row_spans_begin, started at row ya): slot = row0 + ((y - ya) << sws) +
-// (x - xa) as there.
+// (x - xa) * xs as there.
 template <class Fn>
-__device__ __forceinline__ void walk_rect_spans(int xa, int xb, int ya, int yb, int row0, int sws, RowSpans& r,
-                                                Fn&& f)
+__device__ __forceinline__ void walk_rect_spans(int xa, int xb, int ya, int yb, int row0, int sws, int xs,
+                                                RowSpans& r, Fn&& f)
 {
     const float fxa = (float)xa, fxb = (float)xb, fxe = fxb + 1.0f;
     for (int y = ya; y <= yb; ++y, row0 += 1 << sws) {
@@ -525,8 +533,8 @@ __device__ __forceinline__ void walk_rect_spans(int xa, int xb, int ya, int yb, 
         const float lo = fminf(fmaxf(fmaxf(fmaxf(r.L[0], r.L[1]), r.L[2]), fxa), fxe);
         const float hi = fminf(fminf(fminf(r.H[0], r.H[1]), r.H[2]), fxb);
         const float py = (float)y, px0 = ceilf(lo), px1 = floorf(hi);
-        int slot = row0 + ((int)px0 - xa);
-        for (float px = px0; px <= px1; px += 1.0f, ++slot) f(px, py, slot, 1u);
+        int slot = row0 + ((int)px0 - xa) * xs;
+        for (float px = px0; px <= px1; px += 1.0f, slot += xs) f(px, py, slot, 1u);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             r.H[k] += r.s[k];
@@ -536,29 +544,30 @@ __device__ __forceinline__ void walk_rect_spans(int xa, int xb, int ya, int yb, 
 }
 
 // Calls f(px, py, slot, sub) for every logical pixel of the box that lands in
-// the tile's wrapped region (2x - W in [tc0,tc1), row y + 1).
+// the tile's wrapped region (column 2x - W in [tc0,tc1), row y + 1).
 template <class Fn>
 __device__ __forceinline__ void walk_wrapped(const TileGeom& g, int bx0, int bx1, int by0, int by1, Fn&& f)
 {
-    const int wx0 = (g.tc0 + g.W) >> 1;
-    const int xa = max(bx0, wx0), xb = min(bx1, ((g.tc1 + g.W) >> 1) - 1);
+    const int xa = max(bx0, (g.tc0 + g.W + 1) >> 1), xb = min(bx1, (g.tc1 - 1 + g.W) >> 1);
     const int ya = max(by0, g.tr0 - 1), yb = min(by1, g.tr1 - 2);
-    walk_rect(xa, xb, ya, yb, ((ya + 1 - g.tr0) << g.sws) + (xa - wx0), g.sws, 0u, f);
+    walk_rect(xa, xb, ya, yb, ((ya + 1 - g.tr0) << g.sws) + ((2 * xa - g.W - g.tc0) >> g.cs), g.sws, 2 >> g.cs, 0u,
+              f);
 }
 
 // Calls f(px, py, slot, sub) for the candidate pixels of the box that land in
-// the tile: the direct region (2x in [tc0,tc1), row y) with each row cut to
-// its span (spans started at row max(by0, tr0)), then the wrapped one.
+// the tile: the direct region (column 2x in [tc0,tc1), row y) with each row
+// cut to its span (spans started at row max(by0, tr0)), then the wrapped one.
 template <class Fn>
 __device__ __forceinline__ void walk_tile(const TileGeom& g, int bx0, int bx1, int by0, int by1, bool use_spans,
                                           RowSpans& r, Fn&& f)
 {
-    const int xa = max(bx0, g.tc0 >> 1), xb = min(bx1, (g.tc1 >> 1) - 1);
+    const int xa = max(bx0, (g.tc0 + 1) >> 1), xb = min(bx1, (g.tc1 - 1) >> 1);
     const int ya = max(by0, g.tr0), yb = min(by1, g.tr1 - 1);
+    const int row0 = ((ya - g.tr0) << g.sws) + ((2 * xa - g.tc0) >> g.cs);
     if (use_spans)
-        walk_rect_spans(xa, xb, ya, yb, ((ya - g.tr0) << g.sws) + (xa - (g.tc0 >> 1)), g.sws, r, f);
+        walk_rect_spans(xa, xb, ya, yb, row0, g.sws, 2 >> g.cs, r, f);
     else
-        walk_rect(xa, xb, ya, yb, ((ya - g.tr0) << g.sws) + (xa - (g.tc0 >> 1)), g.sws, 1u, f);
+        walk_rect(xa, xb, ya, yb, row0, g.sws, 2 >> g.cs, 1u, f);
     walk_wrapped(g, bx0, bx1, by0, by1, f);
 }
 
@@ -610,17 +619,29 @@ __device__ __forceinline__ uint32_t entry_tri(uint32_t e, const uint4* __restric
 // ONE: every frame of the launch has exactly one draw, so the draw is the
 // same for the whole workgroup and its fields come through scalar loads that
 // do not wait for the list entries.
-// Tile (tx, ty) of frame fr, whose list is pool[o, o + n).
-template <bool BIG, bool ONE>
+//
+// ODD: frames of odd width.  Every cell column holds a z slot (direct pixels
+// on even columns, wrapped ones on odd columns) and a cell's final value is
+// the later write of two slots' winners: its own slot's and the one to its
+// left (rasterizer.rs:82-85 writes cells id and id + 1), and the row's
+// newline for cell 1.  The tile first derives every slot's winner (depth
+// and winner out, the cell into LDS), then resolves each cell whose two
+// slots it holds; a first-column cell takes its own slot's winner here and
+// is finished by k_tile_edges, from the winners this tile and its left
+// neighbour leave in `edge_out` (first and last column, per row).
+// Tile (tx, ty) of frame fr, global tile index gt, whose list is pool[o, o + n).
+template <bool BIG, bool ONE, bool ODD>
 __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, const Ramp* __restrict__ ramps,
-                                            const Ramp* s_ramp, const Frame& fr, uint32_t tx, uint32_t ty, uint64_t o,
-                                            uint32_t n, int tws, int ths, const uint32_t* __restrict__ pool,
+                                            const Ramp* s_ramp, const Frame& fr, uint32_t tx, uint32_t ty, uint64_t gt,
+                                            uint64_t o, uint32_t n, int tws, int ths, const uint32_t* __restrict__ pool,
                                             const int32_t* __restrict__ nl, const uint4* __restrict__ careful_box,
                                             uint32_t* __restrict__ cells, float* __restrict__ zbuf,
-                                            int32_t* __restrict__ win, uint32_t win_raw, uint64_t* smem)
+                                            int32_t* __restrict__ win, uint32_t win_raw, uint4* __restrict__ edge_out,
+                                            uint64_t* smem)
 {
+    static_assert(!(BIG && ODD), "the big-box walk maps slots of even widths only");
     const uint32_t W = fr.W, H = fr.H;
-    const int sws = tws - 1;
+    const int sws = tws - (ODD ? 0 : 1);
     const uint32_t S = 1u << (sws + ths);
     TileGeom g;
     g.tc0 = (int)(tx << tws);
@@ -628,6 +649,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
     g.tr0 = (int)(ty << ths);
     g.tr1 = min((int)H, g.tr0 + (1 << ths));
     g.sws = sws;
+    g.cs = ODD ? 0 : 1;
     g.W = (int)W;
 
     const uint32_t* list = pool + o;
@@ -638,7 +660,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
     // A thread owns the slots of one column of the tile, every row_step-th
     // row (the block is at least one tile row of slots wide).
     const uint32_t my_sc = threadIdx.x & ((1u << sws) - 1);
-    const int my_col = g.tc0 + 2 * (int)my_sc;
+    const int my_col = g.tc0 + ((int)my_sc << g.cs);
     const int row_step = (int)(blockDim.x >> sws);
     const uint64_t id_step = (uint64_t)row_step * W;
     // fn(slot, row, id of the slot's first cell) over this thread's slots
@@ -650,8 +672,21 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         for (; row < g.tr1; row += row_step, id += id_step)
             fn(((uint32_t)(row - g.tr0) << sws) | my_sc, row, id);
     };
-    // Stores the cells and depths of a slot from its winning cell (or blank).
-    auto store = [&](int row, uint64_t id, uint64_t k, uint32_t cell, float z) {
+    // ODD: the winners' cells of the tile's slots, for the per-cell resolve
+    uint32_t* wcell = (uint32_t*)(smem + S) + CHUNK * CACHE_WORDS;
+    // Stores the cells and depths of slot s from its winning cell (or
+    // blank); ODD: the depth and winner, and the cell into wcell.
+    auto store = [&](uint32_t s, int row, uint64_t id, uint64_t k, uint32_t cell, float z) {
+        if constexpr (ODD) {
+            const int32_t wt = (k == KEY_EMPTY) ? -1 : (int32_t)tile_key_tri(k);
+            __builtin_nontemporal_store(k == KEY_EMPTY ? F32_MAX : z, fz + id);
+            if (win) {
+                const int32_t wv = (k == KEY_EMPTY) ? -1 : (win_raw ? (int32_t)((uint32_t)wt << 1 | tile_key_sub(k)) : wt);
+                __builtin_nontemporal_store(wv, win + fr.z_off + id);
+            }
+            if (k != KEY_EMPTY) wcell[s] = cell;
+            return;
+        }
         uint32_t c0 = CELL_BLANK, c1 = CELL_BLANK;
         float z0 = F32_MAX;
         int32_t wt = -1;
@@ -683,7 +718,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
     if (n == 0) {
         // no triangle reaches the tile: blank cells, f32::MAX depths, and the
         // newline cells of column 1
-        for_my_slots([&](uint32_t, int row, uint64_t id) { store(row, id, KEY_EMPTY, CELL_BLANK, F32_MAX); });
+        for_my_slots([&](uint32_t s, int row, uint64_t id) { store(s, row, id, KEY_EMPTY, CELL_BLANK, F32_MAX); });
     } else {
         // glyph of shade sh under shader r: the first LDS_RAMPS shaders from
         // LDS, the others from global memory (two calls, so that each reads
@@ -821,10 +856,10 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                         } else {
                             if (use_spans)
                                 walk_rect_spans(xa + (g.tc0 >> 1), xb + (g.tc0 >> 1), ya + g.tr0, yb + g.tr0,
-                                                (ya << sws) + xa, sws, spans, test);
+                                                (ya << sws) + xa, sws, 1, spans, test);
                             else
                                 walk_rect(xa + (g.tc0 >> 1), xb + (g.tc0 >> 1), ya + g.tr0, yb + g.tr0,
-                                          (ya << sws) + xa, sws, 1u, test);
+                                          (ya << sws) + xa, sws, 1, 1u, test);
                         }
                         walk_wrapped(g, bx0, bx1, by0, by1, test);
                     } else {
@@ -884,7 +919,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                     uint32_t cell;
                     float z;
                     winner(row, k, cell, z);
-                    store(row, id, k, cell, z);
+                    store(s, row, id, k, cell, z);
                     keys[s] = k | TILE_KEY_STORED;
                 });
         }
@@ -894,7 +929,56 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             uint32_t cell = CELL_BLANK;
             float z = F32_MAX;
             if (k != KEY_EMPTY) winner(row, k, cell, z);
-            store(row, id, k, cell, z);
+            store(s, row, id, k, cell, z);
+        });
+    }
+    if constexpr (ODD) {
+        // Each cell: the later write of its own slot's winner and the left
+        // slot's (write time = triangle, then the pixel's row and column,
+        // as the sequential loop orders them), and for cell 1 the row's
+        // newline, which follows its triangle's pixels of that row.
+        const uint64_t* keys = smem;
+        const bool any = n > 0;
+        if (any) __syncthreads();   // every slot's winner cell is in wcell
+        auto slot_time = [&](uint64_t k, int row, int col) -> uint64_t {
+            const uint32_t sub = tile_key_sub(k);
+            const uint32_t py = sub ? (uint32_t)row : (uint32_t)row - 1u;
+            const uint32_t px = sub ? (uint32_t)col >> 1 : (uint32_t)(col + (int)W) >> 1;
+            return (((uint64_t)tile_key_tri(k) << 32) | ((uint64_t)py << 16) | px) + 1;
+        };
+        for_my_slots([&](uint32_t s, int row, uint64_t id) {
+            uint64_t t_own = 0, t_best = 0;
+            uint32_t c_own = CELL_BLANK, best = CELL_BLANK;
+            if (any) {
+                const uint64_t k = keys[s];
+                if (k != KEY_EMPTY) {
+                    t_own = slot_time(k, row, my_col);
+                    c_own = wcell[s];
+                }
+                t_best = t_own;
+                best = c_own;
+                if (my_col > g.tc0) {
+                    const uint64_t kl = keys[s - 1];
+                    if (kl != KEY_EMPTY) {
+                        const uint64_t tl = slot_time(kl, row, my_col - 1);
+                        if (tl > t_best) {
+                            t_best = tl;
+                            best = wcell[s - 1];
+                        }
+                    }
+                }
+            }
+            if (fr.image && my_col == 1) {
+                const int32_t nlr = fnl[row];
+                if (nlr >= 0) {
+                    const uint64_t tn = (((uint64_t)(uint32_t)nlr << 32) | ((uint64_t)row << 16) | 0xFFFFu) + 1;
+                    if (tn > t_best) best = CELL_NEWLINE;
+                }
+            }
+            __builtin_nontemporal_store(best, fc + id);
+            uint4* e = edge_out + (gt * (1u << ths) + (uint64_t)(row - g.tr0)) * 2;
+            if (my_col == g.tc0) e[0] = make_uint4(c_own, 0u, (uint32_t)t_own, (uint32_t)(t_own >> 32));
+            if (my_col == g.tc1 - 1) e[1] = make_uint4(c_own, 0u, (uint32_t)t_own, (uint32_t)(t_own >> 32));
         });
     }
     // image mode: the H cells after the W*H grid are never written (context.rs:38-40)
@@ -909,13 +993,13 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
 // to 7 they fit 72 with SGPR spills to VGPR lanes only, and run ~3% faster.
 // The big-box instance for several draws per frame would spill VGPRs to
 // scratch at 72 and keeps 6.
-template <bool BIG, bool ONE>
+template <bool BIG, bool ONE, bool ODD>
 __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_eu(BIG && !ONE ? 6 : 7, 8))) k_tile_raster(
     const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
     int tws, int ths, const uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ pool, const int32_t* __restrict__ nl, const uint4* __restrict__ careful_box,
     uint32_t* __restrict__ cells, float* __restrict__ zbuf, int32_t* __restrict__ win, uint32_t win_raw,
-    uint32_t n_ramps)
+    uint32_t n_ramps, uint4* __restrict__ edge_out)
 {
     extern __shared__ uint64_t smem[];
     // the job's first LDS_RAMPS shaders, so that a glyph lookup reads LDS
@@ -931,8 +1015,43 @@ __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_
     if (n > 0)
         for (uint32_t i = threadIdx.x; i < min(n_ramps, LDS_RAMPS) * (uint32_t)sizeof(Ramp) / 4; i += blockDim.x)
             ((uint32_t*)s_ramp)[i] = ((const uint32_t*)ramps)[i];
-    raster_tile<BIG, ONE>(draws, ramps, s_ramp, fr, tx, ty, o, n, tws, ths, pool, nl, careful_box, cells, zbuf, win, win_raw,
-                smem);
+    raster_tile<BIG, ONE, ODD>(draws, ramps, s_ramp, fr, tx, ty, gt, o, n, tws, ths, pool, nl, careful_box, cells, zbuf,
+                               win, win_raw, edge_out, smem);
+}
+
+// Frames of odd width: the first-column cells of the tiles.  Cell (r, c)
+// with c the first column of a tile is the later write of slot (r, c)'s
+// winner and slot (r, c - 1)'s, which the tile to the left holds; cell
+// (r, 0) takes the last slot of row r - 1 instead (id + 1 of the row's last
+// direct pixel wraps to the next row).  The tiles left each first and last
+// column slot's winning cell and write time in `edges` (time 0: none).  A
+// frame one column wide has no pixels (the domain's x range is empty), and
+// its cell (r, 0) is row r - 1's newline cell (r - 1) * W + 1.
+// Grid (rows / 256, tile columns, frame - f_lo).
+__global__ void __launch_bounds__(256) k_tile_edges(const Frame* __restrict__ frames, uint32_t f_lo, int tws,
+                                                    int ths, const uint4* __restrict__ edges,
+                                                    const int32_t* __restrict__ nl, uint32_t* __restrict__ cells)
+{
+    const Frame& fr = frames[f_lo + blockIdx.z];
+    const uint32_t b = blockIdx.y;
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= fr.tiles_x || r >= fr.H || (b == 0 && r == 0)) return;
+    const uint32_t TH = 1u << ths;
+    const uint64_t own = fr.tile_begin + (uint64_t)(r >> ths) * fr.tiles_x + b;
+    const uint4 e_own = edges[(own * TH + (r & (TH - 1))) * 2];
+    uint4 e_left;
+    if (b > 0) {
+        e_left = edges[((own - 1) * TH + (r & (TH - 1))) * 2 + 1];
+    } else {
+        const uint32_t rl = r - 1;
+        const uint64_t left = fr.tile_begin + (uint64_t)(rl >> ths) * fr.tiles_x + (fr.tiles_x - 1);
+        e_left = edges[(left * TH + (rl & (TH - 1))) * 2 + 1];
+    }
+    const uint64_t t_own = ((uint64_t)e_own.w << 32) | e_own.z, t_left = ((uint64_t)e_left.w << 32) | e_left.z;
+    uint32_t cell = CELL_BLANK;
+    if (t_own | t_left) cell = t_own > t_left ? e_own.x : e_left.x;
+    if (fr.W == 1 && fr.image && nl[fr.nl_off + r - 1] >= 0) cell = CELL_NEWLINE;
+    cells[fr.cell_off + (uint64_t)r * fr.W + ((uint64_t)b << tws)] = cell;
 }
 
 // ---------------------------------------------------------------------------
@@ -954,18 +1073,24 @@ int TiledPlan::set_tile(uint32_t tw, uint32_t th)
     if (tw != TW || th != TH) sized = false;
     TW = tw;
     TH = th;
+    user_tile = true;
     return SLOTH_OK;
 }
 
 size_t TiledPlan::raster_lds() const
 {
-    // keys (8 B per slot), triangle rows
-    size_t S = (size_t)(TW / 2) * TH;
-    return S * 8 + (size_t)CHUNK * CACHE_WORDS * 4;
+    // keys (8 B per slot), triangle rows; odd widths: a slot per cell and
+    // the slots' winning cells (4 B per slot)
+    size_t S = (size_t)(odd ? TW : TW / 2) * TH;
+    return S * 8 + (size_t)CHUNK * CACHE_WORDS * 4 + (odd ? S * 4 : 0);
 }
 
-int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
+int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows, bool odd_frames)
 {
+    if (odd_frames != odd) sized = false;
+    odd = odd_frames;
+    // odd widths hold twice the slots per tile: half the default height
+    if (!user_tile) TH = odd ? 16 : 32;
     uint64_t tiles = 0;
     for (Frame& f : frames) {
         f.tiles_x = (f.W + TW - 1) / TW;
@@ -982,6 +1107,7 @@ int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
     if ((rc = d_nl.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_rows)))) return rc;
     if ((rc = d_overflow.reserve(sizeof(uint32_t) * std::max<size_t>(1, frames.size())))) return rc;
     if ((rc = d_careful_cnt.reserve(sizeof(uint32_t) * (BIG_CNT + 1)))) return rc;
+    if (odd && (rc = d_edges.reserve(sizeof(uint4) * 2 * std::max<uint64_t>(1, n_tiles) * TH))) return rc;
     return SLOTH_OK;
 }
 
@@ -1110,13 +1236,23 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
         if (mx == 0 || my == 0) continue;
         bool one = true;   // every frame of the range has exactly one draw
         for (uint32_t f = f0; f < f1 && one; ++f) one = frames[f].draw_end - frames[f].draw_begin == 1;
-        const auto kern = walk() ? (one ? k_tile_raster<true, true> : k_tile_raster<true, false>)
-                                 : (one ? k_tile_raster<false, true> : k_tile_raster<false, false>);
+        const auto kern = odd ? (one ? k_tile_raster<false, true, true> : k_tile_raster<false, false, true>)
+                          : walk() ? (one ? k_tile_raster<true, true, false> : k_tile_raster<true, false, false>)
+                                   : (one ? k_tile_raster<false, true, false> : k_tile_raster<false, false, false>);
         hipLaunchKernelGGL(kern, dim3(mx, my, f1 - f0),
                            dim3(RASTER_BLOCK), raster_lds(), s, d_draws, d_ramps, d_frames, f0, log2u(TW), log2u(TH),
                            d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
-                           d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps);
+                           d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps,
+                           odd ? d_edges.as<uint4>() : (uint4*)nullptr);
         HIP_TRY(hipGetLastError());
+        if (odd) {
+            uint32_t mh = 0;
+            for (uint32_t f = f0; f < f1; ++f) mh = std::max(mh, frames[f].H);
+            hipLaunchKernelGGL(k_tile_edges, dim3(blocks(mh, 256), mx, f1 - f0), dim3(256), 0, s, d_frames, f0,
+                               log2u(TW), log2u(TH), (const uint4*)d_edges.as<uint4>(),
+                               (const int32_t*)d_nl.as<int32_t>(), cells);
+            HIP_TRY(hipGetLastError());
+        }
     }
     return SLOTH_OK;
 }
@@ -1127,6 +1263,9 @@ int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>&
                       std::vector<hipEvent_t>* ev, hipStream_t s2, const std::vector<hipEvent_t>* join)
 {
     if (!sized) return fail(SLOTH_ESTATE, "tile lists not sized");
+    if (raster_lds() > 65536)
+        return fail(SLOTH_EINVAL, "tile %ux%u needs %zu bytes of LDS per workgroup (at most 65536)", TW, TH,
+                    raster_lds());
     int rc;
     const uint32_t nf = f_hi - f_lo;
     uint32_t groups = 1;

@@ -18,10 +18,11 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(autouse=True, params=["tiled", "tiled-big", "global"])
 def raster_path(request, monkeypatch):
-    """Every case runs on the tiled path (even widths) with each tile raster
-    variant (SLOTH_BIG_WALK=0: every lane walks its own box; 1: wavefronts
-    walk boxes of more than 128 pixels together) and on the general
-    global-key path (forced by SLOTH_FORCE_GLOBAL=1)."""
+    """Every case runs on the tiled path with each tile raster variant
+    (SLOTH_BIG_WALK=0: every lane walks its own box; 1: wavefronts walk boxes
+    of more than 128 pixels together; frames of odd width take the odd-width
+    instance either way) and on the general global-key path (forced by
+    SLOTH_FORCE_GLOBAL=1)."""
     if request.param == "global":
         monkeypatch.setenv("SLOTH_FORCE_GLOBAL", "1")
     else:
@@ -33,7 +34,7 @@ def raster_path(request, monkeypatch):
 def _check(sloth, name, W, H, U, draws, image=True):
     import os
 
-    path = "global" if os.environ.get("SLOTH_FORCE_GLOBAL") == "1" or W % 2 else "tiled"
+    path = "global" if os.environ.get("SLOTH_FORCE_GLOBAL") == "1" else ("tiled-odd" if W % 2 else "tiled")
     if path == "tiled" and os.environ.get("SLOTH_BIG_WALK") == "1":
         path = "tiled-big"
     say(f"case {name} [{path}]: {W}x{H} image={image} draws={len(draws)} "
@@ -528,3 +529,37 @@ def test_indexed_meshes(sloth):
             for (c0, z0, w0), (c1, z1, w1) in zip(first, out):
                 assert np.array_equal(c0, c1) and np.array_equal(z0.view(np.uint32), z1.view(np.uint32))
                 assert np.array_equal(w0, w1)
+
+
+@pytest.mark.parametrize("tile", [(8, 4), (16, 8), (64, 16), (128, 32)])
+def test_odd_width_tiles(sloth, tile, raster_path):
+    """Frames of odd width on the tiled path: every cell column a z slot,
+    cells resolved from two slots inside a tile, first-column cells finished
+    across tile edges and across rows (cell (r, 0) takes the last slot of
+    row r - 1).  201 and 129 columns leave a last tile one column wide at
+    the 64- and 128-column tiles; terminal-mode frames have no newline
+    cells.  Against the oracle, winners included."""
+    from sloth_amd import meshgen
+
+    if raster_path == "global":
+        pytest.skip("tile sizes only exist on the tiled path")
+    m = meshgen.displaced_icosphere(3, seed=19)
+    sph = meshgen.uv_sphere(16, 8, seed=20)
+    frames, draws, plan = [], [], []
+    sms = [sloth.SimpleMesh.from_mesh(m), sloth.SimpleMesh.from_mesh(sph)]
+    for f, (W, H, image) in enumerate([(201, 64, True), (129, 40, True), (63, 30, False), (81, 25, True)]):
+        U = meshgen.screen_transform(W, H, [m.bbox_max(), sph.bbox_max()])
+        Rs = [meshgen.rotation_euler(0.3 * f, 1.0 + f, 0.2), meshgen.rotation_euler(0.1, 2.0 + f, 0.4)]
+        frames.append(sloth.FrameSpec(W, H, U, image))
+        for sm, R in zip(sms, Rs):
+            draws.append(sloth.DrawSpec(f, sm, R))
+        plan.append((W, H, U, Rs, image))
+    b = sloth.Batch(frames, draws)
+    b.set_tile(*tile)
+    say(f"case odd-width tiles {tile}")
+    b.run()
+    b.sync()
+    for f, (W, H, U, Rs, image) in enumerate(plan):
+        ref = oracle_render(W, H, U, [(m.xyz, m.rgb, Rs[0], None), (sph.xyz, sph.rgb, Rs[1], None)], image=image)
+        cells, z, win = b.read_frame(f, want_winners=True)
+        assert_frame_equal(cells, z, win, ref, W)
