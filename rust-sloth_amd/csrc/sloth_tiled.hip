@@ -182,17 +182,33 @@ __device__ __forceinline__ int wave_max(int v)
     return wave_reduce(v, [](int a, int b) { return max(a, b); });
 }   // the block aggregates appends in a 16 x 16 window of tiles
 
-// Grid: x = blocks of one draw's triangles, y = draw index - d_lo, so the draw,
-// its frame and its matrix are uniform per block (scalar loads).  CLS_BOX
-// triangles are appended to their tiles here; CLS_SPANS / CLS_FULL ones go to
-// the careful list for k_careful.
+// Grid: x = blocks of BIN_TPT * BIN_BLOCK consecutive triangles of one draw,
+// y = draw index - d_lo, so the draw, its frame and its matrix are uniform
+// per block (scalar loads).  CLS_BOX triangles are appended to their tiles
+// here; CLS_SPANS / CLS_FULL ones go to the careful list for k_careful.
 //
-// Appends: a block's 256 consecutive triangles lie close together on screen.
-// Lanes whose box touches at most 2 x 2 tiles and does not wrap (nearly all)
-// count themselves into LDS counters of a 16 x 16 tile window anchored at the
-// block's smallest tile coordinates; one global atomicAdd per window tile then
-// reserves that tile's run, and each lane writes its entry at run + its LDS
-// count.  Other lanes append with one global atomicAdd per tile.
+// Each thread takes BIN_TPT triangles (k * BIN_BLOCK + thread), set up one
+// after another with their loads in flight together, so that the block's
+// barriers and its global atomic round trip serve BIN_TPT times as many
+// triangles (the kernel is bound by latency, not by issue).  Per triangle a
+// thread keeps two words between the phases: its tile rectangle and flags,
+// and its rows.
+//
+// Appends: a block's consecutive triangles lie close together on screen.
+// Triangles whose box touches at most 2 x 2 tiles, all in the direct region
+// or all in the wrapped one (nearly all), count themselves into LDS counters
+// of a 16 x 16 tile window anchored at the block's smallest tile coordinates;
+// one global atomicAdd per window tile then reserves that tile's run (the
+// thread that reserves it also reads the tile's pool range), and each
+// triangle writes its entry at run + its LDS count.  Others append with one
+// global atomicAdd per tile.
+constexpr int BIN_TPT = 4;
+
+// geo word: qx0 (bits 0-13), qy0 (14-27) of the tile rectangle, wide (28),
+// tall (29), CLS_BOX (30); tile coordinates stay below 2^14 (frames of at
+// most 65535 cells a side, tiles of at least 8 x 4 cells).
+constexpr uint32_t GEO_WIDE = 1u << 28, GEO_TALL = 1u << 29, GEO_LOOSE = 1u << 30;
+
 template <bool COUNT>
 __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draws, uint32_t d_lo,
                                                    const Frame* __restrict__ frames, int tws, int ths,
@@ -202,58 +218,20 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
                                                    uint2* __restrict__ careful_in, uint32_t careful_cap)
 {
     __shared__ uint32_t w_cnt[BIN_WIN * BIN_WIN];
-    __shared__ uint32_t w_base[BIN_WIN * BIN_WIN];
+    __shared__ uint64_t w_base[BIN_WIN * BIN_WIN];   // pool index of the tile's reserved run
+    __shared__ uint64_t w_end[BIN_WIN * BIN_WIN];    // end of the tile's pool range
     __shared__ int32_t s_nl[BIN_BLOCK];
     __shared__ int s_tx0, s_ty0, s_lo, s_hi;
 
     const uint32_t d = d_lo + blockIdx.y;
     const Draw& dr = draws[d];
-    const uint32_t base = blockIdx.x * BIN_BLOCK;
+    const uint32_t base = blockIdx.x * (BIN_BLOCK * BIN_TPT);
     if (base >= dr.n) return;
-    const uint32_t local = base + threadIdx.x;
-    const bool valid = local < dr.n;
     const uint32_t frame = dr.frame;
     const Frame& fr = frames[frame];
     const uint32_t W = fr.W, H = fr.H;
-    const uint32_t tri = dr.tri_begin + local;
     const uint64_t tile_begin = fr.tile_begin;
     const uint32_t tiles_x = fr.tiles_x;
-
-    // Lanes past the draw's end set up its last triangle and discard it, so
-    // that the set-up below has no branch around it (the lanes of a wavefront
-    // take different classes; selects cost less than divergent paths).
-    Tri t;
-    setup_draw_tri(dr, valid ? local : dr.n - 1, W, H, t);
-    const uint32_t cls = valid ? setup_class(t) : CLS_EMPTY;
-    const bool boxed = cls == CLS_BOX || cls == CLS_TIGHT;
-    const bool careful = cls == CLS_SPANS || cls == CLS_FULL;
-    const uint32_t entry = tri | (cls == CLS_BOX ? LOOSE_FLAG : 0u);
-    const int y0 = valid ? t.y0 : 0, y1d = valid ? t.y1d : 0;
-    int bx0 = 1, bx1 = 0;
-    if (boxed) box_x(t, cls == CLS_TIGHT, bx0, bx1);
-    TileRects r;
-    tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);   // empty when bx0 > bx1
-    const bool big = COUNT && boxed && (int64_t)(bx1 - bx0 + 1) * (int64_t)(t.y1d - t.y0) > (int64_t)BIG_BOX;
-    const bool has_direct = r.dx0 <= r.dx1 && r.dy0 <= r.dy1;
-    const bool has_wrap = r.wx0 <= r.wx1 && r.wy0 <= r.wy1;
-    const bool small = has_direct && !has_wrap && r.dx1 - r.dx0 <= 1 && r.dy1 - r.dy0 <= 1;
-    const bool wide = r.dx1 > r.dx0, tall = r.dy1 > r.dy0;
-
-    // The pool ranges of a small box's (up to 2 x 2) tiles, loaded now so
-    // that the loads are in flight through the barriers and atomics below
-    // rather than after them.  A row of two tiles needs three offsets.
-    uint64_t o00 = 0, o01 = 0, o02 = 0, o10 = 0, o11 = 0, o12 = 0;
-    if (!COUNT && small) {
-        const uint64_t g0 = tile_begin + (uint64_t)r.dy0 * tiles_x + (uint64_t)r.dx0;
-        o00 = off[g0];
-        o01 = off[g0 + 1];
-        if (wide) o02 = off[g0 + 2];
-        if (tall) {
-            o10 = off[g0 + tiles_x];
-            o11 = off[g0 + tiles_x + 1];
-            if (wide) o12 = off[g0 + tiles_x + 2];
-        }
-    }
 
     w_cnt[threadIdx.x] = 0;
     s_nl[threadIdx.x] = -1;
@@ -261,29 +239,80 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         s_tx0 = s_ty0 = s_lo = 1 << 30;
         s_hi = -1;
     }
-    __syncthreads();
 
-    if (COUNT) {
-        const uint64_t m = __ballot(big);
-        if (__lane_id() == 0 && m) atomicAdd(&careful_cnt[BIG_CNT], (uint32_t)__popcll(m));
-    }
-    if (careful) {
-        const uint32_t idx = atomicAdd(careful_cnt, 1u);
-        if (idx < careful_cap) careful_in[idx] = make_uint2(d, local);
-        else if (!COUNT) overflow[frame] = 1u;
-    }
-    const bool has_rows = fr.image && valid && y0 < y1d;
-    {
-        const int wlo = wave_min(has_rows ? y0 : (1 << 30));
-        const int whi = wave_max(has_rows ? y1d - 1 : -1);
-        const int wtx = wave_min(small ? r.dx0 : (1 << 30));
-        const int wty = wave_min(small ? r.dy0 : (1 << 30));
-        if (__lane_id() == 0) {
-            if (whi >= 0) {
-                atomicMin(&s_lo, wlo);
-                atomicMax(&s_hi, whi);
+    // Phase 1: set-up and classes.  Lanes past the draw's end set up its
+    // last triangle and discard it, so that the set-up has no branch around
+    // it (the lanes of a wavefront take different classes; selects cost less
+    // than divergent paths).
+    uint32_t geo[BIN_TPT], rows[BIN_TPT];
+    uint32_t tiled = 0, small = 0;   // bit k: triangle k has tiles here / has a small rectangle
+    int acc_lo = 1 << 30, acc_hi = -1, acc_tx = 1 << 30, acc_ty = 1 << 30;
+#pragma unroll
+    for (int k = 0; k < BIN_TPT; ++k) {
+        const uint32_t local = base + (uint32_t)k * BIN_BLOCK + threadIdx.x;
+        const bool valid = local < dr.n;
+        Tri t;
+        setup_draw_tri(dr, valid ? local : dr.n - 1, W, H, t);
+        const uint32_t cls = valid ? setup_class(t) : CLS_EMPTY;
+        const bool boxed = cls == CLS_BOX || cls == CLS_TIGHT;
+        const bool careful = cls == CLS_SPANS || cls == CLS_FULL;
+        uint32_t g = cls == CLS_BOX ? GEO_LOOSE : 0u;
+        // A wavefront whose triangles are all back faces or empty (half of
+        // a closed mesh's, in runs) has no tiles to find: a uniform branch.
+        if (__ballot(boxed)) {
+            int bx0 = 1, bx1 = 0;
+            if (boxed) box_x(t, cls == CLS_TIGHT, bx0, bx1);
+            TileRects r;
+            tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);   // empty when bx0 > bx1
+            if (COUNT) {
+                const bool big = boxed && (int64_t)(bx1 - bx0 + 1) * (int64_t)(t.y1d - t.y0) > (int64_t)BIG_BOX;
+                const uint64_t m = __ballot(big);
+                if (__lane_id() == 0 && m) atomicAdd(&careful_cnt[BIG_CNT], (uint32_t)__popcll(m));
             }
-            if (wtx < (1 << 30)) {
+            const bool has_direct = r.dx0 <= r.dx1 && r.dy0 <= r.dy1;
+            const bool has_wrap = r.wx0 <= r.wx1 && r.wy0 <= r.wy1;
+            // A box in one region appends through that region's tile
+            // rectangle [qx0, qx1] x [qy0, qy1]; a box that wraps in part
+            // (it spans 2x = W) takes the per-tile path.  (Meshes wider than
+            // the frame's logical width put whole patches of triangles in
+            // the wrapped region, whose per-tile atomics would meet on few
+            // addresses.)
+            const bool wrap_only = has_wrap && !has_direct;
+            const int qx0 = wrap_only ? r.wx0 : r.dx0, qx1 = wrap_only ? r.wx1 : r.dx1;
+            const int qy0 = wrap_only ? r.wy0 : r.dy0, qy1 = wrap_only ? r.wy1 : r.dy1;
+            const bool sm = has_direct != has_wrap && qx1 - qx0 <= 1 && qy1 - qy0 <= 1;
+            if (has_direct || has_wrap) tiled |= 1u << k;
+            if (sm) {
+                small |= 1u << k;
+                g |= (uint32_t)qx0 | ((uint32_t)qy0 << 14) | (qx1 > qx0 ? GEO_WIDE : 0u) | (qy1 > qy0 ? GEO_TALL : 0u);
+                acc_tx = min(acc_tx, qx0);
+                acc_ty = min(acc_ty, qy0);
+            }
+        }
+        geo[k] = g;
+        if (careful) {
+            const uint32_t idx = atomicAdd(careful_cnt, 1u);
+            if (idx < careful_cap) careful_in[idx] = make_uint2(d, local);
+            else if (!COUNT) overflow[frame] = 1u;
+        }
+        const bool has_rows = fr.image && valid && t.y0 < t.y1d;
+        rows[k] = has_rows ? (uint32_t)t.y0 | ((uint32_t)t.y1d << 16) : 0u;
+        if (has_rows) {
+            acc_lo = min(acc_lo, t.y0);
+            acc_hi = max(acc_hi, t.y1d - 1);
+        }
+    }
+    __syncthreads();   // LDS initialised
+
+    {
+        const int wlo = wave_min(acc_lo), whi = wave_max(acc_hi);
+        if (__lane_id() == 0 && whi >= 0) {
+            atomicMin(&s_lo, wlo);
+            atomicMax(&s_hi, whi);
+        }
+        if (__ballot(small != 0)) {
+            const int wtx = wave_min(acc_tx), wty = wave_min(acc_ty);
+            if (__lane_id() == 0) {
                 atomicMin(&s_tx0, wtx);
                 atomicMin(&s_ty0, wty);
             }
@@ -291,35 +320,53 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     }
     __syncthreads();
 
-    const int ox = s_tx0, oy = s_ty0;
-    const bool inwin = small && r.dx1 - ox < BIN_WIN && r.dy1 - oy < BIN_WIN;
-    const int c = (r.dy0 - oy) * BIN_WIN + (r.dx0 - ox);
-    uint32_t lp0 = 0, lp1 = 0, lp2 = 0, lp3 = 0;
-    if (inwin) {
-        lp0 = atomicAdd(&w_cnt[c], 1u);
-        if (wide) lp1 = atomicAdd(&w_cnt[c + 1], 1u);
-        if (tall) lp2 = atomicAdd(&w_cnt[c + BIN_WIN], 1u);
-        if (wide && tall) lp3 = atomicAdd(&w_cnt[c + BIN_WIN + 1], 1u);
-    }
+    // Phase 2: window counts and newline rows in LDS.
     // rasterizer.rs:89-91: a newline cell after every row of the domain; the
     // last triangle (in draw order) whose rows include r decides row r.
     // The block reduces its rows in LDS, one global atomicMax per row: here
     // when they span fewer than BIN_BLOCK rows, else in windows of BIN_BLOCK
     // rows at the end (tall triangles put every row of a tall frame under
     // the atomics of each triangle that covers it).
-    const int lo = s_lo, hi = s_hi;
+    const int ox = s_tx0, oy = s_ty0, lo = s_lo, hi = s_hi;
     const bool nl_lds = hi >= lo && hi - lo < BIN_BLOCK;
-    int32_t* fnl = nl + fr.nl_off;
-    if (has_rows && nl_lds)
-        for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)tri);
+    auto in_window = [&](uint32_t g, int& c) {
+        const int qx0 = (int)(g & 0x3FFFu), qy0 = (int)((g >> 14) & 0x3FFFu);
+        c = (qy0 - oy) * BIN_WIN + (qx0 - ox);
+        return qx0 + (int)((g >> 28) & 1u) - ox < BIN_WIN && qy0 + (int)((g >> 29) & 1u) - oy < BIN_WIN;
+    };
+    const uint32_t tri0 = dr.tri_begin + base + threadIdx.x;
+    uint32_t lpa[BIN_TPT], lpb[BIN_TPT];   // LDS counts: tiles (0,0) | (1,0) << 16, (0,1) | (1,1) << 16
+#pragma unroll
+    for (int k = 0; k < BIN_TPT; ++k) {
+        lpa[k] = lpb[k] = 0;
+        int c;
+        if (((small >> k) & 1u) && in_window(geo[k], c)) {
+            const bool wide = geo[k] & GEO_WIDE, tall = geo[k] & GEO_TALL;
+            uint32_t l0 = atomicAdd(&w_cnt[c], 1u), l1 = 0, l2 = 0, l3 = 0;
+            if (wide) l1 = atomicAdd(&w_cnt[c + 1], 1u);
+            if (tall) l2 = atomicAdd(&w_cnt[c + BIN_WIN], 1u);
+            if (wide && tall) l3 = atomicAdd(&w_cnt[c + BIN_WIN + 1], 1u);
+            lpa[k] = l0 | (l1 << 16);
+            lpb[k] = l2 | (l3 << 16);
+        }
+        if (nl_lds && rows[k]) {
+            const int y0 = (int)(rows[k] & 0xFFFFu), y1d = (int)(rows[k] >> 16);
+            for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)(tri0 + (uint32_t)k * BIN_BLOCK));
+        }
+    }
     __syncthreads();
 
+    int32_t* fnl = nl + fr.nl_off;
     {
         const uint32_t n = w_cnt[threadIdx.x];
         if (n) {
             const uint64_t gt = tile_begin + (uint64_t)(oy + (int)(threadIdx.x / BIN_WIN)) * tiles_x +
                                 (uint64_t)(ox + (int)(threadIdx.x % BIN_WIN));
-            w_base[threadIdx.x] = atomicAdd(&cursor[gt], n);
+            const uint32_t pos = atomicAdd(&cursor[gt], n);
+            if (!COUNT) {
+                w_base[threadIdx.x] = off[gt] + pos;
+                w_end[threadIdx.x] = off[gt + 1];
+            }
         }
     }
     if (nl_lds) {
@@ -328,27 +375,54 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     }
     __syncthreads();
 
-    if (inwin) {
-        if (!COUNT) {
-            // tile (dx0 + i, dy0 + j) holds pool[o_ji, o_j(i+1))
-            auto put = [&](uint64_t o, uint64_t e, uint32_t pos) {
-                if (pos < e - o) pool[o + pos] = entry;
-                else overflow[frame] = 1u;
-            };
-            put(o00, o01, w_base[c] + lp0);
-            if (wide) put(o01, o02, w_base[c + 1] + lp1);
-            if (tall) put(o10, o11, w_base[c + BIN_WIN] + lp2);
-            if (wide && tall) put(o11, o12, w_base[c + BIN_WIN + 1] + lp3);
-        }
-    } else if (has_direct || has_wrap) {
-        // large or wrapping boxes: one global atomic per tile
-        TileIter it;
-        iter_begin(r, it);
-        while (it.phase < 2) {
-            const uint64_t gt = tile_begin + (uint64_t)it.ty * tiles_x + (uint64_t)it.tx;
-            const uint32_t pos = atomicAdd(&cursor[gt], 1u);
-            put_entry<COUNT>(gt, pos, entry, off, pool, overflow, frame);
-            iter_next(r, it);
+    // Phase 3: entries.
+#pragma unroll
+    for (int k = 0; k < BIN_TPT; ++k) {
+        if (!((tiled >> k) & 1u)) continue;
+        const uint32_t local = base + (uint32_t)k * BIN_BLOCK + threadIdx.x;
+        const uint32_t entry = (dr.tri_begin + local) | ((geo[k] & GEO_LOOSE) ? LOOSE_FLAG : 0u);
+        int c;
+        if ((small >> k) & 1u) {
+            const bool wide = geo[k] & GEO_WIDE, tall = geo[k] & GEO_TALL;
+            if (in_window(geo[k], c)) {
+                if (!COUNT) {
+                    auto put = [&](int cc, uint32_t l) {
+                        const uint64_t p = w_base[cc] + l;
+                        if (p < w_end[cc]) pool[p] = entry;
+                        else overflow[frame] = 1u;
+                    };
+                    put(c, lpa[k] & 0xFFFFu);
+                    if (wide) put(c + 1, lpa[k] >> 16);
+                    if (tall) put(c + BIN_WIN, lpb[k] & 0xFFFFu);
+                    if (wide && tall) put(c + BIN_WIN + 1, lpb[k] >> 16);
+                }
+            } else {
+                // a small rectangle outside the window: one global atomic per tile
+                const int qx0 = (int)(geo[k] & 0x3FFFu), qy0 = (int)((geo[k] >> 14) & 0x3FFFu);
+                for (int j = 0; j <= (int)tall; ++j)
+                    for (int i = 0; i <= (int)wide; ++i) {
+                        const uint64_t gt = tile_begin + (uint64_t)(qy0 + j) * tiles_x + (uint64_t)(qx0 + i);
+                        const uint32_t pos = atomicAdd(&cursor[gt], 1u);
+                        put_entry<COUNT>(gt, pos, entry, off, pool, overflow, frame);
+                    }
+            }
+        } else {
+            // large or part-wrapping boxes (few): set up again, one global
+            // atomic per tile
+            Tri t;
+            setup_draw_tri(dr, local, W, H, t);
+            int bx0, bx1;
+            box_x(t, !(geo[k] & GEO_LOOSE), bx0, bx1);
+            TileRects r;
+            tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);
+            TileIter it;
+            iter_begin(r, it);
+            while (it.phase < 2) {
+                const uint64_t gt = tile_begin + (uint64_t)it.ty * tiles_x + (uint64_t)it.tx;
+                const uint32_t pos = atomicAdd(&cursor[gt], 1u);
+                put_entry<COUNT>(gt, pos, entry, off, pool, overflow, frame);
+                iter_next(r, it);
+            }
         }
     }
     if (hi >= lo && !nl_lds) {   // uniform per block
@@ -356,9 +430,13 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
             __syncthreads();   // the previous window's reads are done
             s_nl[threadIdx.x] = -1;
             __syncthreads();
-            if (has_rows)
+#pragma unroll
+            for (int k = 0; k < BIN_TPT; ++k) {
+                if (!rows[k]) continue;
+                const int y0 = (int)(rows[k] & 0xFFFFu), y1d = (int)(rows[k] >> 16);
                 for (int y = max(y0, ws); y < min(y1d, ws + BIN_BLOCK); ++y)
-                    atomicMax(&s_nl[y - ws], (int32_t)tri);
+                    atomicMax(&s_nl[y - ws], (int32_t)(tri0 + (uint32_t)k * BIN_BLOCK));
+            }
             __syncthreads();
             const int32_t v = s_nl[threadIdx.x];
             if (ws + (int)threadIdx.x <= hi && v >= 0) atomicMax(&fnl[ws + (int)threadIdx.x], v);
@@ -1136,7 +1214,7 @@ int TiledPlan::launch_bin(const std::vector<Frame>& frames, const std::vector<Dr
         uint32_t max_n = 0;
         for (uint32_t d = d0; d < d1; ++d) max_n = std::max(max_n, draws[d].n);
         if (max_n == 0) continue;
-        dim3 grid(blocks(max_n, BIN_BLOCK), d1 - d0);
+        dim3 grid(blocks(max_n, BIN_BLOCK * BIN_TPT), d1 - d0);
         if (count_only)
             hipLaunchKernelGGL(k_bin<true>, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
                                d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
