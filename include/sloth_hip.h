@@ -287,21 +287,31 @@ int sloth_batch_set_tile(sloth_batch *b, uint32_t tile_w, uint32_t tile_h);
  * in at least 1 triangle of 512).  Results are identical either way. */
 int sloth_batch_set_walk(sloth_batch *b, int mode);
 
+/* Triangles per thread of the binning kernel (1, 2 or 4) for the batch's
+ * next renders: more keeps more loads and atomics in flight per barrier,
+ * fewer spreads small draws over more workgroups.  Results are identical
+ * either way. */
+int sloth_batch_set_bin(sloth_batch *b, uint32_t tris_per_thread);
+
 /* Renders the batch with each candidate tile size -- 8..64 logical pixels
  * a side, i.e. 16, 32, 64 or 128 cells wide by 8, 16, 32 or 64 rows -- timed
- * with HIP events over 3 graph renders each, keeps the fastest, then times
+ * with HIP events as the least of three runs of 3 graph renders each, keeps
+ * the fastest, then times
  * the tile raster with and without the wavefront walk of big boxes at that
- * size and keeps the faster.  The choice is remembered for batches of the
- * same class (frame sizes, cell count and log2 of the triangle count) in this
+ * size and keeps the faster, then the binning kernel with 4, 2 and 1
+ * triangles per thread.  The choice is remembered for batches of the same
+ * class (frame sizes, cell count and log2 of the triangle count) in this
  * process and reused without timing.  Writes the chosen size to tile_w /
  * tile_h (may be NULL; 0 for batches on the general path, which has no
  * tiles). */
 int sloth_batch_autotune(sloth_batch *b, uint32_t *tile_w, uint32_t *tile_h);
 
 /* The batch's current raster plan: tile size in cells (0 x 0 on the general
- * path) and whether the tile raster walks big boxes by wavefront (1), not
- * (0), or -1 on the general path. */
-int sloth_batch_plan(const sloth_batch *b, uint32_t *tile_w, uint32_t *tile_h, int *big_walk);
+ * path), whether the tile raster walks big boxes by wavefront (1), not (0),
+ * or -1 on the general path, and the binning kernel's triangles per thread
+ * (0 on the general path).  Any pointer may be NULL. */
+int sloth_batch_plan(const sloth_batch *b, uint32_t *tile_w, uint32_t *tile_h, int *big_walk,
+                     uint32_t *bin_tris_per_thread);
 
 int sloth_batch_destroy(sloth_batch *b);
 

@@ -1385,6 +1385,15 @@ int sloth_batch_set_walk(sloth_batch* b, int mode)
     return SLOTH_OK;
 }
 
+int sloth_batch_set_bin(sloth_batch* b, uint32_t tris_per_thread)
+{
+    ARG_CHECK(b, "null batch");
+    ARG_CHECK(tris_per_thread == 1 || tris_per_thread == 2 || tris_per_thread == 4,
+              "tris_per_thread must be 1, 2 or 4");
+    b->job.tiled.set_bin_tpt((int)tris_per_thread);
+    return SLOTH_OK;
+}
+
 }  // extern "C"
 
 // Autotuner cache: class of a batch -> best tile size.
@@ -1395,6 +1404,7 @@ int sloth_batch_set_walk(sloth_batch* b, int mode)
 struct TuneChoice {
     uint32_t tw, th;
     int walk;
+    int bin_tpt;
 };
 static std::mutex g_tune_mu;
 static std::map<std::tuple<uint64_t, uint64_t, uint32_t>, TuneChoice> g_tune;
@@ -1430,6 +1440,7 @@ int sloth_batch_autotune(sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h)
         int r = sloth_batch_set_tile(b, c.tw, c.th);
         if (r) return r;
         j.tiled.set_walk(c.walk);
+        j.tiled.set_bin_tpt(c.bin_tpt);
         if (tile_w) *tile_w = c.tw;
         if (tile_h) *tile_h = c.th;
         return SLOTH_OK;
@@ -1441,16 +1452,24 @@ int sloth_batch_autotune(sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h)
         if (it != g_tune.end()) return apply(it->second);
     }
     // Tiles of 8..64 logical pixels a side (a logical pixel is two cells
-    // wide): 16..128 cells by 8..64 rows.  Each candidate is timed over 3
-    // graph renders after one untimed render (which also sizes the lists).
+    // wide): 16..128 cells by 8..64 rows.  Each candidate is timed as the
+    // least of three runs of 3 graph renders, after one untimed render (which
+    // also sizes the lists): single runs differ by a few per cent, as much
+    // as neighbouring candidates do.
     auto time_it = [&](float& ms) -> int {
         int r = sloth_batch_time(b, 3, 1, 0, &ms);
+        for (int k = 0; k < 2 && r == SLOTH_OK; ++k) {
+            float m2 = 0.0f;
+            r = sloth_batch_time(b, 3, 0, 0, &m2);
+            ms = std::min(ms, m2);
+        }
         return r;
     };
-    TuneChoice best{j.tiled.TW, j.tiled.TH, -1};
+    TuneChoice best{j.tiled.TW, j.tiled.TH, -1, 4};
     float best_ms = 1e30f;
     static const uint32_t widths[] = {16, 32, 64, 128}, heights[] = {8, 16, 32, 64};
     j.tiled.set_walk(-1);
+    j.tiled.set_bin_tpt(4);
     for (uint32_t tw : widths)
         for (uint32_t th : heights) {
             if ((rc = sloth_batch_set_tile(b, tw, th))) return rc;
@@ -1459,7 +1478,7 @@ int sloth_batch_autotune(sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h)
             if ((rc = time_it(ms))) return rc;
             if (ms < best_ms) {
                 best_ms = ms;
-                best = TuneChoice{tw, th, -1};
+                best = TuneChoice{tw, th, -1, 4};
             }
         }
     // then the raster variant: the sizing run's choice against the other
@@ -1467,11 +1486,26 @@ int sloth_batch_autotune(sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h)
     {
         float ms = 0.0f;
         if ((rc = time_it(ms))) return rc;   // sizes the lists; the plan's own choice
+        best_ms = std::min(ms, best_ms);
         const int other = j.tiled.big_walk ? 0 : 1;
         j.tiled.set_walk(other);
         float ms2 = 0.0f;
         if ((rc = time_it(ms2))) return rc;
-        best.walk = ms2 < std::min(ms, best_ms) ? other : -1;
+        if (ms2 < best_ms) {
+            best.walk = other;
+            best_ms = ms2;
+        }
+        j.tiled.set_walk(best.walk);
+    }
+    // then the bin's triangles per thread
+    for (int tpt : {2, 1}) {
+        j.tiled.set_bin_tpt(tpt);
+        float ms = 0.0f;
+        if ((rc = time_it(ms))) return rc;
+        if (ms < best_ms) {
+            best.bin_tpt = tpt;
+            best_ms = ms;
+        }
     }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1480,7 +1514,8 @@ int sloth_batch_autotune(sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h)
     return apply(best);
 }
 
-int sloth_batch_plan(const sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h, int* big_walk)
+int sloth_batch_plan(const sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h, int* big_walk,
+                     uint32_t* bin_tris_per_thread)
 {
     ARG_CHECK(b, "null batch");
     const Job& j = b->job;
@@ -1488,6 +1523,7 @@ int sloth_batch_plan(const sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h, i
     if (tile_w) *tile_w = tiled ? j.tiled.TW : 0;
     if (tile_h) *tile_h = tiled ? j.tiled.TH : 0;
     if (big_walk) *big_walk = tiled ? (j.tiled.walk() ? 1 : 0) : -1;
+    if (bin_tris_per_thread) *bin_tris_per_thread = tiled ? (uint32_t)j.tiled.bin_tpt : 0;
     return SLOTH_OK;
 }
 

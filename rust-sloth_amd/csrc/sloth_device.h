@@ -178,18 +178,25 @@ __device__ __forceinline__ float depth(const Tri& t, float w1, float w2)
 // z component of Triangle::normal() after Unit::new_normalize
 // (geometry.rs:49-56): products rounded before the subtraction, the squared
 // norm summed as ((x^2 + z^2) + (y^2 + 0^2)), then z / sqrt.
-__device__ __forceinline__ float unit_normal_z(const Tri& t)
+// normal_parts gives nz and the squared norm n2, unit_normal_z = nz / sqrt(n2).
+__device__ __forceinline__ void normal_parts(const Tri& t, float& nz, float& n2)
 {
     float e1x = t.x2 - t.x1, e1y = t.y2 - t.y1, e1z = t.z2 - t.z1;
     float e2x = t.x3 - t.x1, e2y = t.y3 - t.y1, e2z = t.z3 - t.z1;
     float nx = (e1y * e2z) - (e1z * e2y);
     float ny = (e1z * e2x) - (e1x * e2z);
-    float nz = (e1x * e2y) - (e1y * e2x);
+    nz = (e1x * e2y) - (e1y * e2x);
     float a = nx * nx, b = ny * ny, c = nz * nz;
     float d = 0.0f;
     a += c;
     b += d;
-    float n2 = 0.0f + (a + b);
+    n2 = 0.0f + (a + b);
+}
+
+__device__ __forceinline__ float unit_normal_z(const Tri& t)
+{
+    float nz, n2;
+    normal_parts(t, nz, n2);
     return nz / sqrtf(n2);
 }
 

@@ -88,6 +88,13 @@ struct TiledPlan {
         walk_force = mode;
     }
     bool walk() const { return !odd && (walk_force >= 0 ? walk_force == 1 : big_walk); }
+    // Triangles per thread of k_bin: 1, 2 or 4 (set by the autotuner).
+    int bin_tpt = 4;
+    void set_bin_tpt(int t)
+    {
+        if (t != bin_tpt) ++generation;   // captured graphs hold the old launch
+        bin_tpt = t;
+    }
     size_t raster_lds() const;
 
     // Counting run over all frames (synchronous): per-tile list lengths ->

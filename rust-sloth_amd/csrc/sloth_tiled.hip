@@ -182,15 +182,16 @@ __device__ __forceinline__ int wave_max(int v)
     return wave_reduce(v, [](int a, int b) { return max(a, b); });
 }   // the block aggregates appends in a 16 x 16 window of tiles
 
-// Grid: x = blocks of BIN_TPT * BIN_BLOCK consecutive triangles of one draw,
+// Grid: x = blocks of TPT * BIN_BLOCK consecutive triangles of one draw,
 // y = draw index - d_lo, so the draw, its frame and its matrix are uniform
 // per block (scalar loads).  CLS_BOX triangles are appended to their tiles
 // here; CLS_SPANS / CLS_FULL ones go to the careful list for k_careful.
 //
-// Each thread takes BIN_TPT triangles (k * BIN_BLOCK + thread), set up one
-// after another with their loads in flight together, so that the block's
-// barriers and its global atomic round trip serve BIN_TPT times as many
-// triangles (the kernel is bound by latency, not by issue).  Per triangle a
+// Each thread takes TPT triangles (k * BIN_BLOCK + thread), set up one after
+// another with their loads in flight together, so that the block's barriers
+// and its global atomic round trip serve TPT times as many triangles (the
+// kernel is bound by latency, not by issue).  TPT is 1, 2 or 4
+// (TiledPlan::bin_tpt; the autotuner picks it).  Per triangle a
 // thread keeps two words between the phases: its tile rectangle and flags,
 // and its rows.
 //
@@ -202,14 +203,12 @@ __device__ __forceinline__ int wave_max(int v)
 // thread that reserves it also reads the tile's pool range), and each
 // triangle writes its entry at run + its LDS count.  Others append with one
 // global atomicAdd per tile.
-constexpr int BIN_TPT = 4;
-
 // geo word: qx0 (bits 0-13), qy0 (14-27) of the tile rectangle, wide (28),
 // tall (29), CLS_BOX (30); tile coordinates stay below 2^14 (frames of at
 // most 65535 cells a side, tiles of at least 8 x 4 cells).
 constexpr uint32_t GEO_WIDE = 1u << 28, GEO_TALL = 1u << 29, GEO_LOOSE = 1u << 30;
 
-template <bool COUNT>
+template <bool COUNT, int TPT>
 __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draws, uint32_t d_lo,
                                                    const Frame* __restrict__ frames, int tws, int ths,
                                                    uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
@@ -225,7 +224,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 
     const uint32_t d = d_lo + blockIdx.y;
     const Draw& dr = draws[d];
-    const uint32_t base = blockIdx.x * (BIN_BLOCK * BIN_TPT);
+    const uint32_t base = blockIdx.x * (BIN_BLOCK * TPT);
     if (base >= dr.n) return;
     const uint32_t frame = dr.frame;
     const Frame& fr = frames[frame];
@@ -244,11 +243,11 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     // last triangle and discard it, so that the set-up has no branch around
     // it (the lanes of a wavefront take different classes; selects cost less
     // than divergent paths).
-    uint32_t geo[BIN_TPT], rows[BIN_TPT];
+    uint32_t geo[TPT], rows[TPT];
     uint32_t tiled = 0, small = 0;   // bit k: triangle k has tiles here / has a small rectangle
     int acc_lo = 1 << 30, acc_hi = -1, acc_tx = 1 << 30, acc_ty = 1 << 30;
 #pragma unroll
-    for (int k = 0; k < BIN_TPT; ++k) {
+    for (int k = 0; k < TPT; ++k) {
         const uint32_t local = base + (uint32_t)k * BIN_BLOCK + threadIdx.x;
         const bool valid = local < dr.n;
         Tri t;
@@ -335,9 +334,9 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         return qx0 + (int)((g >> 28) & 1u) - ox < BIN_WIN && qy0 + (int)((g >> 29) & 1u) - oy < BIN_WIN;
     };
     const uint32_t tri0 = dr.tri_begin + base + threadIdx.x;
-    uint32_t lpa[BIN_TPT], lpb[BIN_TPT];   // LDS counts: tiles (0,0) | (1,0) << 16, (0,1) | (1,1) << 16
+    uint32_t lpa[TPT], lpb[TPT];   // LDS counts: tiles (0,0) | (1,0) << 16, (0,1) | (1,1) << 16
 #pragma unroll
-    for (int k = 0; k < BIN_TPT; ++k) {
+    for (int k = 0; k < TPT; ++k) {
         lpa[k] = lpb[k] = 0;
         int c;
         if (((small >> k) & 1u) && in_window(geo[k], c)) {
@@ -377,7 +376,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 
     // Phase 3: entries.
 #pragma unroll
-    for (int k = 0; k < BIN_TPT; ++k) {
+    for (int k = 0; k < TPT; ++k) {
         if (!((tiled >> k) & 1u)) continue;
         const uint32_t local = base + (uint32_t)k * BIN_BLOCK + threadIdx.x;
         const uint32_t entry = (dr.tri_begin + local) | ((geo[k] & GEO_LOOSE) ? LOOSE_FLAG : 0u);
@@ -431,7 +430,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
             s_nl[threadIdx.x] = -1;
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < BIN_TPT; ++k) {
+            for (int k = 0; k < TPT; ++k) {
                 if (!rows[k]) continue;
                 const int y0 = (int)(rows[k] & 0xFFFFu), y1d = (int)(rows[k] >> 16);
                 for (int y = max(y0, ws); y < min(y1d, ws + BIN_BLOCK); ++y)
@@ -819,8 +818,16 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         // monotone, so equal indices at the two ends mean one glyph for the
         // whole box.  Shaders that are not non-decreasing, and non-finite
         // values, take the per-pixel evaluation.
-        auto box_glyph = [&](uint32_t r, const Tri& t, float na, int bx0, int bx1, int by0, int by1) -> uint32_t {
-            const Ramp& rp = r < LDS_RAMPS ? s_ramp[r] : ramps[r];
+        //
+        // na comes here from the hardware reciprocal square root: na_h =
+        // fl(fl(nz rsq(n2)) a), rsq within 1 ulp, so for n2 in [2^-100,
+        // 2^100] na_h lies within 2^-21 |na| of the reference's fl(fl(nz /
+        // sqrt(n2)) a) (two roundings each side).  Every fl(na S) then lies
+        // between fl(na_h (o - E')) and fl(na_h (o + E')) with E' = E +
+        // 2^-19 (|o| + E).  The lower end's step index i is counted, and the
+        // upper end compared with threshold i: both ends have index i when
+        // it is at most thr[i] (thresholds are non-decreasing).
+        auto box_glyph = [&](const Ramp& rp, const Tri& t, float na, int bx0, int bx1, int by0, int by1) -> uint32_t {
             if (!rp.sorted) return ~0u;
             const float X0 = (float)bx0, X1 = (float)bx1, Y0 = (float)by0, Y1 = (float)by1;
             const float Sb =
@@ -830,13 +837,15 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
             const float ey = t.y3 - t.y1, ex = t.x3 - t.x1;
             const float o = t.A2 * ey - t.B2 * ex;
             const float E = 5.9604645e-07f * (Sb + fabsf(t.A2) * fabsf(ey) + fabsf(t.B2) * fabsf(ex));
-            const float s1 = na * (o - E), s2 = na * (o + E);
+            const float E2 = E + 0x1p-19f * (fabsf(o) + E);
+            const float s1 = na * (o - E2), s2 = na * (o + E2);
             // NaN or infinity anywhere fails the finite test (v_maximum3 propagates NaN)
             const float mag = __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(na), fabsf(s1)), fabsf(s2));
             if (!(mag < INFINITY)) return ~0u;
-            const uint32_t i1 = ramp_index_sorted(rp, s1), i2 = ramp_index_sorted(rp, s2);
-            if (i1 != i2) return ~0u;
-            return i1 < rp.n ? rp.glyph[i1] : rp.fallback;
+            const float lo = fminf(s1, s2), hi = fmaxf(s1, s2);
+            const uint32_t i = ramp_index_sorted(rp, lo);
+            if (i < rp.n && !(hi <= rp.thr[i])) return ~0u;
+            return i < rp.n ? rp.glyph[i] : rp.fallback;
         };
         uint64_t* keys = smem;
         float* cache = (float*)(keys + S);
@@ -889,7 +898,8 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 const uint32_t li = T - dr.tri_begin;
                 Tri t;
                 setup_draw_tri(dr, li, W, H, t);
-                const float na = unit_normal_z(t) * t.a;   // rasterizer.rs:76 multiplies these first
+                float nz, n2;
+                normal_parts(t, nz, n2);
                 const auto* c = global_ptr(dr.rgb + (size_t)li * 3);
                 const uint32_t rgb = ((uint32_t)c[0] << 8) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 24);
                 if (!given) {
@@ -899,8 +909,17 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 }
                 // a triangle with one glyph over its box keeps its finished
                 // cell where the shade factor would be, flagged by 0xFF
-                const uint32_t bg = box_glyph(dr.ramp, t, na, bx0, bx1, by0, by1);
-                const float w2 = bg != ~0u ? __uint_as_float(bg | rgb) : na;
+                uint32_t bg = ~0u;
+                if (n2 >= 0x1p-100f && n2 <= 0x1p100f) {
+                    const float na_h = nz * __builtin_amdgcn_rsqf(n2) * t.a;
+                    const uint32_t r = dr.ramp;
+                    bg = r < LDS_RAMPS ? box_glyph(s_ramp[r], t, na_h, bx0, bx1, by0, by1)
+                                       : box_glyph(ramps[r], t, na_h, bx0, bx1, by0, by1);
+                }
+                float w2 = __uint_as_float(bg | rgb);
+                // the others keep the shade factor (rasterizer.rs:76
+                // multiplies the unit normal's z by a first)
+                if (bg == ~0u) w2 = nz / sqrtf(n2) * t.a;
                 const uint32_t w3 = rgb | (bg != ~0u ? 0xFFu : dr.ramp);
                 float4* row4 = (float4*)(cache + p * CACHE_WORDS);
                 row4[0] = make_float4(t.x1, t.y1, t.x2, t.y2);
@@ -1214,17 +1233,16 @@ int TiledPlan::launch_bin(const std::vector<Frame>& frames, const std::vector<Dr
         uint32_t max_n = 0;
         for (uint32_t d = d0; d < d1; ++d) max_n = std::max(max_n, draws[d].n);
         if (max_n == 0) continue;
-        dim3 grid(blocks(max_n, BIN_BLOCK * BIN_TPT), d1 - d0);
-        if (count_only)
-            hipLaunchKernelGGL(k_bin<true>, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
+        dim3 grid(blocks(max_n, BIN_BLOCK * (uint32_t)bin_tpt), d1 - d0);
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
                                d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
                                d_nl.as<int32_t>(), d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(),
                                d_careful_in.as<uint2>(), careful_cap);
-        else
-            hipLaunchKernelGGL(k_bin<false>, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
-                               d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
-                               d_nl.as<int32_t>(), d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(),
-                               d_careful_in.as<uint2>(), careful_cap);
+        };
+        if (bin_tpt == 1) count_only ? go(k_bin<true, 1>) : go(k_bin<false, 1>);
+        else if (bin_tpt == 2) count_only ? go(k_bin<true, 2>) : go(k_bin<false, 2>);
+        else count_only ? go(k_bin<true, 4>) : go(k_bin<false, 4>);
         HIP_TRY(hipGetLastError());
     }
     return SLOTH_OK;

@@ -114,7 +114,8 @@ SIGNATURES = {
     "sloth_batch_set_tile": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
     "sloth_batch_set_walk": (C.c_int, [_vp, C.c_int]),
     "sloth_batch_autotune": (C.c_int, [_vp, _u32p, _u32p]),
-    "sloth_batch_plan": (C.c_int, [_vp, _u32p, _u32p, _i32p]),
+    "sloth_batch_plan": (C.c_int, [_vp, _u32p, _u32p, _i32p, _u32p]),
+    "sloth_batch_set_bin": (C.c_int, [_vp, C.c_uint32]),
     "sloth_batch_destroy": (C.c_int, [_vp]),
     "sloth_shading_create": (C.c_int, [_vp, _u32p, C.c_uint32, C.c_uint32, _vpp]),
     "sloth_shading_visibility": (C.c_int, [_vp]),
@@ -607,10 +608,16 @@ class Batch:
         return int(w.value), int(h.value)
 
     def plan(self) -> dict:
-        """The raster plan in use: tile size in cells and the big-box walk."""
-        w, h, bw = C.c_uint32(), C.c_uint32(), C.c_int32()
-        _check(load_library().sloth_batch_plan(self._h, C.byref(w), C.byref(h), C.byref(bw)))
-        return {"tile_w": int(w.value), "tile_h": int(h.value), "big_walk": int(bw.value)}
+        """The raster plan in use: tile size in cells, the big-box walk and
+        the binning kernel's triangles per thread."""
+        w, h, bw, tpt = C.c_uint32(), C.c_uint32(), C.c_int32(), C.c_uint32()
+        _check(load_library().sloth_batch_plan(self._h, C.byref(w), C.byref(h), C.byref(bw), C.byref(tpt)))
+        return {"tile_w": int(w.value), "tile_h": int(h.value), "big_walk": int(bw.value),
+                "bin_tris_per_thread": int(tpt.value)}
+
+    def set_bin(self, tris_per_thread: int) -> None:
+        """Triangles per thread of the binning kernel: 1, 2 or 4."""
+        _check(load_library().sloth_batch_set_bin(self._h, tris_per_thread))
 
     def profile(self) -> dict:
         cap = 16

@@ -6,7 +6,10 @@ the shader at every pixel of the triangle's candidate box; if so, the
 winning slot's cell comes from the entry without re-evaluating the pixel.
 The decision bounds every pixel's edge-function sum by the FP32 orient o
 plus or minus E = 10u (Sb + Q), Sb the sum of |A| |py - ay| + |B| |px - ax|
-over the box's corners and Q the orient's own terms (u = 2^-24).
+over the box's corners and Q the orient's own terms (u = 2^-24).  It takes
+the shade factor from a hardware reciprocal square root (within 1 ulp), so
+it widens E by 2^-19 (|o| + E), counts the step index of the lower end and
+compares the upper end with that step's threshold.
 
 Here that decision is restated in float32 numpy (one rounding per
 operation, as on the GPU) for triangles whose unit normal z sits within a
@@ -18,6 +21,7 @@ so the decision is tested where it matters."""
 from __future__ import annotations
 
 import numpy as np
+import pytest
 
 f32 = np.float32
 THR = np.array([0.20, 0.30, 0.40, 0.50, 0.60, 0.70, 0.80, 0.90, 1.0], np.float32)
@@ -73,12 +77,21 @@ def _setup(t, W, H):
     bx0 = np.maximum(x0, np.ceil(minx - f32(0.03125)).astype(np.int64))
     bx1 = np.minimum(x1d - 1, np.floor(maxx + f32(0.03125)).astype(np.int64))
     return dict(x1=x1, y1=y1, x2=x2, y2=y2, x3=x3, y3=y3, A0=A0, B0=B0, A1=A1, B1=B1, A2=A2, B2=B2, o=o, na=na,
-                bx0=bx0, bx1=bx1, by0=y0, by1=y1d - 1)
+                nz=nz, n2=n2, a=a, bx0=bx0, bx1=bx1, by0=y0, by1=y1d - 1)
 
 
-def _box_glyph_index(s):
-    """The raster's decision (box_glyph in sloth_tiled.hip): the constant step
-    index, or -1."""
+def _na_hardware(s, eps):
+    """na as the raster takes it for the decision: fl(fl(nz * rsq(n2)) * a),
+    with the reciprocal square root off by eps (relative; the hardware's is
+    within 1 ulp)."""
+    with np.errstate(all="ignore"):
+        r = (1.0 / np.sqrt(s["n2"].astype(np.float64)) * (1.0 + eps)).astype(np.float32)
+        return (s["nz"] * r) * s["a"]
+
+
+def _box_glyph_index(s, na_h):
+    """The raster's decision (box_glyph in sloth_tiled.hip) from the shade
+    factor na_h: the constant step index, or -1."""
     X0, X1 = s["bx0"].astype(np.float32), s["bx1"].astype(np.float32)
     Y0, Y1 = s["by0"].astype(np.float32), s["by1"].astype(np.float32)
 
@@ -91,19 +104,24 @@ def _box_glyph_index(s):
     ey, ex = s["y3"] - s["y1"], s["x3"] - s["x1"]
     o = s["A2"] * ey - s["B2"] * ex
     E = f32(5.9604645e-07) * ((Sb + np.abs(s["A2"]) * np.abs(ey)) + np.abs(s["B2"]) * np.abs(ex))
+    E2 = E + f32(2.0 ** -19) * (np.abs(o) + E)
     with np.errstate(all="ignore"):
-        s1, s2 = s["na"] * (o - E), s["na"] * (o + E)
-        mag = np.maximum(np.abs(s["na"]), np.maximum(np.abs(s1), np.abs(s2)))
-    i1, i2 = _index(s1), _index(s2)
-    ok = np.isfinite(mag) & (i1 == i2)
-    return np.where(ok, i1, -1)
+        s1, s2 = na_h * (o - E2), na_h * (o + E2)
+        mag = np.maximum(np.abs(na_h), np.maximum(np.abs(s1), np.abs(s2)))
+    lo, hi = np.minimum(s1, s2), np.maximum(s1, s2)
+    i = _index(lo)
+    ok = np.isfinite(mag) & ((i >= len(THR)) | (hi <= THR[np.minimum(i, len(THR) - 1)]))
+    return np.where(ok, i, -1)
 
 
-def test_constant_glyph_decision_holds_per_pixel():
+@pytest.mark.parametrize("eps", [-2.0 ** -23, 0.0, 2.0 ** -23])
+def test_constant_glyph_decision_holds_per_pixel(eps):
+    """The decision from a reciprocal square root 1 ulp low, exact, and 1
+    ulp high, against the per-pixel shade of the reference's exact na."""
     rng = np.random.default_rng(2024)
     t, W, H, near = _triangles(40000, rng)
     s = _setup(t, W, H)
-    claim = _box_glyph_index(s)
+    claim = _box_glyph_index(s, _na_hardware(s, eps))
     lo_idx = np.full(len(t), 99)
     hi_idx = np.full(len(t), -1)
     bw = (s["bx1"] - s["bx0"] + 1).max()

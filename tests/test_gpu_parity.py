@@ -563,3 +563,48 @@ def test_odd_width_tiles(sloth, tile, raster_path):
         ref = oracle_render(W, H, U, [(m.xyz, m.rgb, Rs[0], None), (sph.xyz, sph.rgb, Rs[1], None)], image=image)
         cells, z, win = b.read_frame(f, want_winners=True)
         assert_frame_equal(cells, z, win, ref, W)
+
+
+@pytest.mark.parametrize("tpt", [1, 2, 4])
+def test_bin_tris_per_thread(sloth, tpt, raster_path):
+    """The binning kernel's triangles per thread (1, 2 or 4) change only how
+    the work is spread: dense meshes in small frames (many triangles per
+    tile), a mesh wider than the frame's logical width (whole patches of
+    triangles in the wrapped region), two draws in one frame and draws whose
+    sizes are not multiples of a block's triangles, against the oracle."""
+    from sloth_amd import meshgen
+
+    if raster_path == "global":
+        pytest.skip("the binning kernel belongs to the tiled path")
+    dense = meshgen.displaced_icosphere(5, seed=31)      # 20,480 triangles
+    box = meshgen.subdivided_box(9, seed=32)             # 972 triangles
+    ball = meshgen.uv_sphere(40, 21, seed=33)
+    spec = [((80, 24), [(dense, 1.0)]), ((200, 64), [(box, 1.7)]), ((64, 64), [(ball, 1.0), (dense, 0.6)])]
+    frames, draws, keep, plan = [], [], [], []
+    rng = np.random.default_rng(30)
+    for f, ((W, H), items) in enumerate(spec):
+        U = meshgen.screen_transform(W, H, [m.bbox_max() for m, _ in items])
+        frames.append(sloth.FrameSpec(W, H, U, True))
+        fd = []
+        for m, zoom in items:
+            R = meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3))
+            R = (R.reshape(4, 4) * np.float32(zoom)).astype(np.float32).reshape(16)
+            R[15] = 1.0
+            sm = sloth.SimpleMesh.from_mesh(m)
+            keep.append(sm)
+            draws.append(sloth.DrawSpec(f, sm, R))
+            fd.append((m.xyz, m.rgb, R, None))
+        plan.append((W, H, U, fd))
+    b = sloth.Batch(frames, draws)
+    b.set_bin(tpt)
+    assert b.plan()["bin_tris_per_thread"] == tpt
+    say(f"case bin triangles per thread {tpt}")
+    for rep in range(2):
+        b.run()
+        b.sync()
+        for f, (W, H, U, fd) in enumerate(plan):
+            ref = oracle_render(W, H, U, fd)
+            cells, z, _ = b.read_frame(f)
+            assert_frame_equal(cells, z, None, ref, W)
+    cells, z, win = b.read_frame(1, want_winners=True)
+    assert_frame_equal(cells, z, win, oracle_render(*plan[1][:3], plan[1][3]), plan[1][0])
