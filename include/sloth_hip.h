@@ -277,8 +277,12 @@ int sloth_batch_profile(sloth_batch *b, float *out_ms, const char **names, uint3
  * milliseconds of the timed renders. */
 int sloth_batch_time(sloth_batch *b, uint32_t steps, uint32_t warmup, int mode, float *ms);
 
-/* Tile size of the batch's even-width frames, in cells (width a power of two
- * in 8..128, height in 4..64).  Takes effect at the next render. */
+/* Tile size of the batch's frames, in cells (width a power of two in 8..512,
+ * height in 4..64).  Takes effect at the next render, which fails with
+ * SLOTH_EINVAL when a tile does not fit the raster's workgroup: at most 64 KiB
+ * of LDS (12 KiB of triangle rows, 8 bytes a z slot, 12 for odd widths) and
+ * at most 256 z slots a tile row (a slot covers two cells of an even-width
+ * frame and one of an odd-width frame). */
 int sloth_batch_set_tile(sloth_batch *b, uint32_t tile_w, uint32_t tile_h);
 
 /* Tile raster variant of the batch's next renders: 1 = wavefronts walk the
@@ -293,13 +297,12 @@ int sloth_batch_set_walk(sloth_batch *b, int mode);
  * either way. */
 int sloth_batch_set_bin(sloth_batch *b, uint32_t tris_per_thread);
 
-/* Renders the batch with each candidate tile size -- 8..64 logical pixels
- * a side, i.e. 16, 32, 64 or 128 cells wide by 8, 16, 32 or 64 rows -- timed
- * with HIP events as the least of three runs of 3 graph renders each, keeps
- * the fastest, then times
- * the tile raster with and without the wavefront walk of big boxes at that
- * size and keeps the faster, then the binning kernel with 4, 2 and 1
- * triangles per thread.  The choice is remembered for batches of the same
+/* Renders the batch with each candidate tile size that fits -- 16, 32, 64,
+ * 128, 256 or 512 cells (8..256 logical pixels) wide by 8, 16, 32 or 64 rows
+ * -- timed with HIP events as the least of three runs of 3 graph renders
+ * each, keeps the fastest, then times the tile raster with and without the
+ * wavefront walk of big boxes at that size and keeps the faster, then the
+ * binning kernel with 4, 2 and 1 triangles per thread.  The choice is remembered for batches of the same
  * class (frame sizes, cell count and log2 of the triangle count) in this
  * process and reused without timing.  Writes the chosen size to tile_w /
  * tile_h (may be NULL; 0 for batches on the general path, which has no

@@ -1451,8 +1451,9 @@ int sloth_batch_autotune(sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h)
         auto it = g_tune.find(key);
         if (it != g_tune.end()) return apply(it->second);
     }
-    // Tiles of 8..64 logical pixels a side (a logical pixel is two cells
-    // wide): 16..128 cells by 8..64 rows.  Each candidate is timed as the
+    // Tiles of 8..256 logical pixels wide (a logical pixel is two cells
+    // wide) by 8..64 rows: 16..512 cells by 8..64 rows, those the raster's
+    // workgroup can take.  Each candidate is timed as the
     // least of three runs of 3 graph renders, after one untimed render (which
     // also sizes the lists): single runs differ by a few per cent, as much
     // as neighbouring candidates do.
@@ -1467,13 +1468,13 @@ int sloth_batch_autotune(sloth_batch* b, uint32_t* tile_w, uint32_t* tile_h)
     };
     TuneChoice best{j.tiled.TW, j.tiled.TH, -1, 4};
     float best_ms = 1e30f;
-    static const uint32_t widths[] = {16, 32, 64, 128}, heights[] = {8, 16, 32, 64};
+    static const uint32_t widths[] = {16, 32, 64, 128, 256, 512}, heights[] = {8, 16, 32, 64};
     j.tiled.set_walk(-1);
     j.tiled.set_bin_tpt(4);
     for (uint32_t tw : widths)
         for (uint32_t th : heights) {
             if ((rc = sloth_batch_set_tile(b, tw, th))) return rc;
-            if (j.tiled.raster_lds() > 65536) continue;   // odd widths: too many slots per tile
+            if (!j.tiled.tile_fits()) continue;   // too many slots per tile or per row
             float ms = 0.0f;
             if ((rc = time_it(ms))) return rc;
             if (ms < best_ms) {

@@ -79,7 +79,7 @@ struct TiledPlan {
     // every frame has an odd width.
     int plan(std::vector<Frame>& frames, uint64_t rows, bool odd_frames);
 
-    // Tile size in cells: TW a power of two in 8..128, TH in 4..64.
+    // Tile size in cells: TW a power of two in 8..512, TH in 4..64.
     int set_tile(uint32_t tw, uint32_t th);
     // The raster variant the next launches use (-1 = the sizing run's choice).
     void set_walk(int mode)
@@ -96,6 +96,8 @@ struct TiledPlan {
         bin_tpt = t;
     }
     size_t raster_lds() const;
+    // The tile raster can take the current tile size (LDS, slots per row).
+    bool tile_fits() const;
 
     // Counting run over all frames (synchronous): per-tile list lengths ->
     // pool offsets with headroom.

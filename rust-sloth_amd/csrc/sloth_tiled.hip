@@ -1165,8 +1165,8 @@ static int log2u(uint32_t v)
 
 int TiledPlan::set_tile(uint32_t tw, uint32_t th)
 {
-    if (tw < 8 || tw > 128 || th < 4 || th > 64 || (tw & (tw - 1)) || (th & (th - 1)))
-        return fail(SLOTH_EINVAL, "tile size %ux%u: width must be a power of two in 8..128, height in 4..64", tw, th);
+    if (tw < 8 || tw > 512 || th < 4 || th > 64 || (tw & (tw - 1)) || (th & (th - 1)))
+        return fail(SLOTH_EINVAL, "tile size %ux%u: width must be a power of two in 8..512, height in 4..64", tw, th);
     if (tw != TW || th != TH) sized = false;
     TW = tw;
     TH = th;
@@ -1180,6 +1180,13 @@ size_t TiledPlan::raster_lds() const
     // the slots' winning cells (4 B per slot)
     size_t S = (size_t)(odd ? TW : TW / 2) * TH;
     return S * 8 + (size_t)CHUNK * CACHE_WORDS * 4 + (odd ? S * 4 : 0);
+}
+
+bool TiledPlan::tile_fits() const
+{
+    // the raster's threads own whole columns of slots: a tile row holds at
+    // most one workgroup's worth (odd widths: a slot per cell)
+    return raster_lds() <= 65536 && (odd ? TW : TW / 2) <= (uint32_t)RASTER_BLOCK;
 }
 
 int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows, bool odd_frames)
@@ -1359,9 +1366,10 @@ int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>&
                       std::vector<hipEvent_t>* ev, hipStream_t s2, const std::vector<hipEvent_t>* join)
 {
     if (!sized) return fail(SLOTH_ESTATE, "tile lists not sized");
-    if (raster_lds() > 65536)
-        return fail(SLOTH_EINVAL, "tile %ux%u needs %zu bytes of LDS per workgroup (at most 65536)", TW, TH,
-                    raster_lds());
+    if (!tile_fits())
+        return fail(SLOTH_EINVAL,
+                    "tile %ux%u does not fit a workgroup: %zu bytes of LDS (at most 65536), %u slots a row "
+                    "(at most %d)", TW, TH, raster_lds(), odd ? TW : TW / 2, RASTER_BLOCK);
     int rc;
     const uint32_t nf = f_hi - f_lo;
     uint32_t groups = 1;

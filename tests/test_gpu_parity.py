@@ -386,7 +386,7 @@ def test_empty_mesh_and_frame(sloth):
     _check(sloth, "no draws", W, H, I, [])
 
 
-@pytest.mark.parametrize("tile", [(32, 8), (128, 32), (8, 4), (64, 64)])
+@pytest.mark.parametrize("tile", [(32, 8), (128, 32), (8, 4), (64, 64), (256, 8), (512, 4), (512, 16)])
 def test_tile_sizes_agree(sloth, tile, raster_path):
     """Every tile size gives the oracle's frames (the wrap and newline cells
     land in different tiles for different sizes)."""
@@ -427,7 +427,7 @@ def test_autotune_keeps_results(sloth, raster_path):
     b = sloth.Batch([sloth.FrameSpec(W, H, U, True) for _ in Rs], [sloth.DrawSpec(k, sm, R) for k, R in enumerate(Rs)])
     tw, th = b.autotune()
     say(f"case autotune -> {tw}x{th}")
-    assert tw in (16, 32, 64, 128) and th in (8, 16, 32, 64)
+    assert tw in (16, 32, 64, 128, 256, 512) and th in (8, 16, 32, 64)
     assert b.plan()["tile_w"] == tw and b.plan()["tile_h"] == th
     b.run()
     b.sync()
@@ -531,7 +531,7 @@ def test_indexed_meshes(sloth):
                 assert np.array_equal(w0, w1)
 
 
-@pytest.mark.parametrize("tile", [(8, 4), (16, 8), (64, 16), (128, 32)])
+@pytest.mark.parametrize("tile", [(8, 4), (16, 8), (64, 16), (128, 32), (256, 8)])
 def test_odd_width_tiles(sloth, tile, raster_path):
     """Frames of odd width on the tiled path: every cell column a z slot,
     cells resolved from two slots inside a tile, first-column cells finished
@@ -608,3 +608,28 @@ def test_bin_tris_per_thread(sloth, tpt, raster_path):
             assert_frame_equal(cells, z, None, ref, W)
     cells, z, win = b.read_frame(1, want_winners=True)
     assert_frame_equal(cells, z, win, oracle_render(*plan[1][:3], plan[1][3]), plan[1][0])
+
+
+def test_tile_that_does_not_fit(sloth, raster_path):
+    """A tile the raster's workgroup cannot take (512 slots a row on an
+    odd-width frame, or more than 64 KiB of LDS) is refused at the render
+    with SLOTH_EINVAL before any launch; a fitting size then renders."""
+    from sloth_amd import meshgen
+
+    if raster_path == "global":
+        pytest.skip("tile sizes only exist on the tiled path")
+    m = meshgen.uv_sphere(16, 8, seed=40)
+    W, H = 101, 40
+    U = meshgen.screen_transform(W, H, [m.bbox_max()])
+    R = meshgen.rotation_euler(0.2, 0.7, 0.1)
+    b = sloth.Batch([sloth.FrameSpec(W, H, U, True)], [sloth.DrawSpec(0, sloth.SimpleMesh.from_mesh(m), R)])
+    for tile in [(512, 8), (256, 64)]:
+        b.set_tile(*tile)
+        with pytest.raises(sloth.SlothError) as e:
+            b.run()
+        assert e.value.code == sloth.SLOTH_EINVAL
+    b.set_tile(64, 16)
+    b.run()
+    b.sync()
+    cells, z, win = b.read_frame(0, want_winners=True)
+    assert_frame_equal(cells, z, win, oracle_render(W, H, U, [(m.xyz, m.rgb, R, None)]), W)
