@@ -215,7 +215,10 @@ typedef struct sloth_draw_desc {
 /* Builds a batch: every frame cleared, then its draws in array order (draws
  * of one frame need not be contiguous; their relative order is the frame's
  * draw order).  Output buffers for all frames are allocated in device memory.
- * shader = NULL means default_shader. */
+ * shader = NULL means default_shader.  Frames of any widths render on the
+ * tiled path; a batch with a frame of 2^22 - 1 or more triangles takes the
+ * general path (per-slot 64-bit keys in device memory), which has no tiles.
+ * Results are identical either way. */
 int sloth_batch_create(const sloth_frame_desc *frames, uint32_t n_frames, const sloth_draw_desc *draws,
                        uint32_t n_draws, const sloth_ramp *shader, sloth_batch **out);
 

@@ -219,7 +219,7 @@ struct Job {
         draws.assign(din.size(), Draw{});
         uint64_t cells = 0, z = 0, rows = 0, items = 0;
         size_t k = 0;
-        bool all_even = true, all_odd = true, small_frames = true;
+        bool small_frames = true;
         for (uint32_t f = 0; f < F; ++f) {
             Frame& fr = frames[f];
             fr.W = fin[f].W;
@@ -261,19 +261,15 @@ struct Job {
             cells = (cells + 63) & ~63ull;
             z += WH;
             rows += fr.H;
-            if (fr.W & 1u) all_even = false;
-            else all_odd = false;
         }
         n_cells = cells;
         n_z = z;
         n_rows = rows;
         n_items = items;
         // the tiled path keeps a draw's shader index in the low byte of an
-        // LDS word, where 0xFF marks a triangle whose glyph is constant
-        // frames of one width parity share a tile raster instance; a batch that
-        // mixes odd and even widths takes the general path
-        path = ((all_even || all_odd) && small_frames && ramps.size() < 256 && !force_global()) ? PATH_TILED
-                                                                                               : PATH_GLOBAL;
+        // LDS word, where 0xFF marks a triangle whose glyph is constant;
+        // frames of either width parity (or both) take it
+        path = (small_frames && ramps.size() < 256 && !force_global()) ? PATH_TILED : PATH_GLOBAL;
         int rc;
         if ((rc = d_frames.reserve(sizeof(Frame) * std::max<size_t>(1, frames.size())))) return rc;
         if ((rc = d_draws.reserve(sizeof(Draw) * std::max<size_t>(1, draws.size())))) return rc;
@@ -284,7 +280,7 @@ struct Job {
             if ((rc = d_keys.reserve(sizeof(uint64_t) * n_z))) return rc;
             if ((rc = d_nl.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_rows)))) return rc;
         } else {
-            if ((rc = tiled.plan(frames, n_rows, all_odd && !frames.empty()))) return rc;
+            if ((rc = tiled.plan(frames, n_rows))) return rc;
         }
         laid_out = true;
         return SLOTH_OK;

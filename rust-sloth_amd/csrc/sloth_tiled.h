@@ -1,9 +1,10 @@
-// Tiled raster path for frames of even width (internal).
+// Tiled raster path (internal).
 //
 // The W x H cell grid of every frame is cut into TW x TH cell tiles (TW even,
 // tiles start on even columns).  With W even, z slot ids = y*W + 2x are even,
 // so a tile's slots and the two cells each slot writes (id, id+1) never cross
-// a tile edge, and a tile can resolve its cells alone.
+// a tile edge, and a tile can resolve its cells alone.  Frames of odd width
+// take the raster's odd instance (described there).
 //
 //   k_tile_clear   tile cursors <- 0, newline rows <- -1, overflow <- 0,
 //                  careful-list length <- 0
@@ -52,9 +53,11 @@ constexpr uint32_t PIPE_MIN_FRAMES = 16;  // frames per group at least
 struct TiledPlan {
     uint32_t TW = 64, TH = 32;       // tile size in cells (TW even)
     bool user_tile = false;          // set_tile was called (keeps the size on a re-plan)
-    // every frame has an odd width: a z slot on every cell column, cells
-    // resolved from two slots, first-column cells finished by k_tile_edges
-    bool odd = false;
+    // Frames of even / odd width in the job.  Each parity has its raster
+    // instance (a launch skips the other parity's frames); odd widths hold a
+    // z slot on every cell column, resolve cells from two slots and finish
+    // first-column cells in k_tile_edges.
+    bool has_even = true, has_odd = false;
     DevBuf d_edges;                  // odd: per tile and row, the first and last column slots' winners
     uint64_t n_tiles = 0;
     uint64_t n_rows = 0;
@@ -75,9 +78,8 @@ struct TiledPlan {
     int walk_force = -1;
 
     // Sets the tile grid of every frame and allocates per-tile and per-row
-    // state.  Pool offsets become invalid until size() runs.  odd_frames:
-    // every frame has an odd width.
-    int plan(std::vector<Frame>& frames, uint64_t rows, bool odd_frames);
+    // state.  Pool offsets become invalid until size() runs.
+    int plan(std::vector<Frame>& frames, uint64_t rows);
 
     // Tile size in cells: TW a power of two in 8..512, TH in 4..64.
     int set_tile(uint32_t tw, uint32_t th);
@@ -87,7 +89,8 @@ struct TiledPlan {
         if (mode != walk_force) ++generation;   // captured graphs hold the old variant
         walk_force = mode;
     }
-    bool walk() const { return !odd && (walk_force >= 0 ? walk_force == 1 : big_walk); }
+    // the even-width instance's variant (the odd one has no big-box walk)
+    bool walk() const { return has_even && (walk_force >= 0 ? walk_force == 1 : big_walk); }
     // Triangles per thread of k_bin: 1, 2 or 4 (set by the autotuner).
     int bin_tpt = 4;
     void set_bin_tpt(int t)
@@ -95,8 +98,10 @@ struct TiledPlan {
         if (t != bin_tpt) ++generation;   // captured graphs hold the old launch
         bin_tpt = t;
     }
-    size_t raster_lds() const;
-    // The tile raster can take the current tile size (LDS, slots per row).
+    // LDS of the even- or odd-width raster instance at the current tile size
+    size_t raster_lds(bool odd) const;
+    // Both parities' instances the job needs can take the current tile size
+    // (LDS, slots per row).
     bool tile_fits() const;
 
     // Counting run over all frames (synchronous): per-tile list lengths ->

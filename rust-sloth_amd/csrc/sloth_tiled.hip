@@ -1105,7 +1105,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_
 
     const Frame& fr = frames[f_lo + blockIdx.z];
     const uint32_t tx = blockIdx.x, ty = blockIdx.y;
-    if (tx >= fr.tiles_x || ty >= fr.tiles_y) return;
+    if (tx >= fr.tiles_x || ty >= fr.tiles_y || ((fr.W & 1u) != 0) != ODD) return;
     const uint64_t gt = fr.tile_begin + (uint64_t)ty * fr.tiles_x + tx;
     const uint64_t o = off[gt];
     const uint32_t n = min(cursor[gt], (uint32_t)(off[gt + 1] - o));
@@ -1132,7 +1132,7 @@ __global__ void __launch_bounds__(256) k_tile_edges(const Frame* __restrict__ fr
     const Frame& fr = frames[f_lo + blockIdx.z];
     const uint32_t b = blockIdx.y;
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= fr.tiles_x || r >= fr.H || (b == 0 && r == 0)) return;
+    if (!(fr.W & 1u) || b >= fr.tiles_x || r >= fr.H || (b == 0 && r == 0)) return;
     const uint32_t TH = 1u << ths;
     const uint64_t own = fr.tile_begin + (uint64_t)(r >> ths) * fr.tiles_x + b;
     const uint4 e_own = edges[(own * TH + (r & (TH - 1))) * 2];
@@ -1174,7 +1174,7 @@ int TiledPlan::set_tile(uint32_t tw, uint32_t th)
     return SLOTH_OK;
 }
 
-size_t TiledPlan::raster_lds() const
+size_t TiledPlan::raster_lds(bool odd) const
 {
     // keys (8 B per slot), triangle rows; odd widths: a slot per cell and
     // the slots' winning cells (4 B per slot)
@@ -1186,15 +1186,19 @@ bool TiledPlan::tile_fits() const
 {
     // the raster's threads own whole columns of slots: a tile row holds at
     // most one workgroup's worth (odd widths: a slot per cell)
-    return raster_lds() <= 65536 && (odd ? TW : TW / 2) <= (uint32_t)RASTER_BLOCK;
+    auto fits = [&](bool odd) { return raster_lds(odd) <= 65536 && (odd ? TW : TW / 2) <= (uint32_t)RASTER_BLOCK; };
+    return (!has_even || fits(false)) && (!has_odd || fits(true));
 }
 
-int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows, bool odd_frames)
+int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
 {
-    if (odd_frames != odd) sized = false;
-    odd = odd_frames;
+    bool ev = false, od = false;
+    for (const Frame& f : frames) ((f.W & 1u) ? od : ev) = true;
+    if (ev != has_even || od != has_odd) sized = false;
+    has_even = ev;
+    has_odd = od;
     // odd widths hold twice the slots per tile: half the default height
-    if (!user_tile) TH = odd ? 16 : 32;
+    if (!user_tile) TH = has_odd ? 16 : 32;
     uint64_t tiles = 0;
     for (Frame& f : frames) {
         f.tiles_x = (f.W + TW - 1) / TW;
@@ -1211,7 +1215,7 @@ int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows, bool odd_frames)
     if ((rc = d_nl.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_rows)))) return rc;
     if ((rc = d_overflow.reserve(sizeof(uint32_t) * std::max<size_t>(1, frames.size())))) return rc;
     if ((rc = d_careful_cnt.reserve(sizeof(uint32_t) * (BIG_CNT + 1)))) return rc;
-    if (odd && (rc = d_edges.reserve(sizeof(uint4) * 2 * std::max<uint64_t>(1, n_tiles) * TH))) return rc;
+    if (has_odd && (rc = d_edges.reserve(sizeof(uint4) * 2 * std::max<uint64_t>(1, n_tiles) * TH))) return rc;
     return SLOTH_OK;
 }
 
@@ -1338,17 +1342,29 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
         }
         if (mx == 0 || my == 0) continue;
         bool one = true;   // every frame of the range has exactly one draw
-        for (uint32_t f = f0; f < f1 && one; ++f) one = frames[f].draw_end - frames[f].draw_begin == 1;
-        const auto kern = odd ? (one ? k_tile_raster<false, true, true> : k_tile_raster<false, false, true>)
-                          : walk() ? (one ? k_tile_raster<true, true, false> : k_tile_raster<true, false, false>)
-                                   : (one ? k_tile_raster<false, true, false> : k_tile_raster<false, false, false>);
-        hipLaunchKernelGGL(kern, dim3(mx, my, f1 - f0),
-                           dim3(RASTER_BLOCK), raster_lds(), s, d_draws, d_ramps, d_frames, f0, log2u(TW), log2u(TH),
-                           d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
-                           d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps,
-                           odd ? d_edges.as<uint4>() : (uint4*)nullptr);
-        HIP_TRY(hipGetLastError());
-        if (odd) {
+        bool ev = false, od = false;
+        for (uint32_t f = f0; f < f1; ++f) {
+            one = one && frames[f].draw_end - frames[f].draw_begin == 1;
+            ((frames[f].W & 1u) ? od : ev) = true;
+        }
+        // one launch per width parity in the range; each skips the other's frames
+        if (ev) {
+            const auto kern = walk() ? (one ? k_tile_raster<true, true, false> : k_tile_raster<true, false, false>)
+                                     : (one ? k_tile_raster<false, true, false> : k_tile_raster<false, false, false>);
+            hipLaunchKernelGGL(kern, dim3(mx, my, f1 - f0), dim3(RASTER_BLOCK), raster_lds(false), s, d_draws,
+                               d_ramps, d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
+                               d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
+                               d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps, (uint4*)nullptr);
+            HIP_TRY(hipGetLastError());
+        }
+        if (od) {
+            const auto kern = one ? k_tile_raster<false, true, true> : k_tile_raster<false, false, true>;
+            hipLaunchKernelGGL(kern, dim3(mx, my, f1 - f0), dim3(RASTER_BLOCK), raster_lds(true), s, d_draws,
+                               d_ramps, d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
+                               d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
+                               d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps,
+                               d_edges.as<uint4>());
+            HIP_TRY(hipGetLastError());
             uint32_t mh = 0;
             for (uint32_t f = f0; f < f1; ++f) mh = std::max(mh, frames[f].H);
             hipLaunchKernelGGL(k_tile_edges, dim3(blocks(mh, 256), mx, f1 - f0), dim3(256), 0, s, d_frames, f0,
@@ -1369,7 +1385,7 @@ int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>&
     if (!tile_fits())
         return fail(SLOTH_EINVAL,
                     "tile %ux%u does not fit a workgroup: %zu bytes of LDS (at most 65536), %u slots a row "
-                    "(at most %d)", TW, TH, raster_lds(), odd ? TW : TW / 2, RASTER_BLOCK);
+                    "(at most %d)", TW, TH, raster_lds(has_odd), has_odd ? TW : TW / 2, RASTER_BLOCK);
     int rc;
     const uint32_t nf = f_hi - f_lo;
     uint32_t groups = 1;

@@ -633,3 +633,36 @@ def test_tile_that_does_not_fit(sloth, raster_path):
     b.sync()
     cells, z, win = b.read_frame(0, want_winners=True)
     assert_frame_equal(cells, z, win, oracle_render(W, H, U, [(m.xyz, m.rgb, R, None)]), W)
+
+
+@pytest.mark.parametrize("tile", [(16, 8), (64, 16), (256, 8)])
+def test_mixed_parity_batch(sloth, tile, raster_path):
+    """A batch that mixes even and odd widths (1 and 2 columns included)
+    stays on the tiled path, one raster instance per parity, and matches
+    the oracle frame by frame, winners included."""
+    from sloth_amd import meshgen
+
+    if raster_path == "global":
+        pytest.skip("tile sizes only exist on the tiled path")
+    m = meshgen.displaced_icosphere(3, seed=41)
+    sm = sloth.SimpleMesh.from_mesh(m)
+    sizes = [(200, 64, True), (201, 64, True), (81, 25, False), (64, 30, True), (1, 5, True), (2, 3, True),
+             (33, 12, True)]
+    frames, draws, plan = [], [], []
+    for f, (W, H, image) in enumerate(sizes):
+        U = meshgen.screen_transform(W, H, [m.bbox_max()])
+        R = meshgen.rotation_euler(0.2 * f, 0.9 + f, 0.1)
+        frames.append(sloth.FrameSpec(W, H, U, image))
+        draws.append(sloth.DrawSpec(f, sm, R))
+        plan.append((W, H, U, R, image))
+    b = sloth.Batch(frames, draws)
+    b.set_tile(*tile)
+    say(f"case mixed-parity batch, tile {tile}")
+    for rep in range(2):
+        b.run()
+        b.sync()
+        assert b.plan()["tile_w"] == tile[0]
+        for f, (W, H, U, R, image) in enumerate(plan):
+            ref = oracle_render(W, H, U, [(m.xyz, m.rgb, R, None)], image=image)
+            cells, z, win = b.read_frame(f, want_winners=(rep == 1))
+            assert_frame_equal(cells, z, win if rep == 1 else None, ref, W)

@@ -63,8 +63,8 @@ def test_visibility_matches_oracle(sloth):
 
 
 # 100 x 70 and 101 x 45: backward tiles and 8-entry runs cut by the frame's
-# right and bottom edges (101 is odd: its visibility comes from the global-key
-# raster)
+# right and bottom edges (101 is odd: its visibility comes from the odd-width
+# tile raster)
 @pytest.mark.parametrize("views,W,H", [(4, 128, 96), (2, 256, 256), (3, 100, 70), (2, 101, 45)])
 def test_gradients_match_fp64_adjoint(sloth, views, W, H):
     m, U, Rs, batch, vc, vn, L, rng = setup_views(sloth, 48, 24, views, W, H, seed=11)
