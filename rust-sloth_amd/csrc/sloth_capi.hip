@@ -1547,6 +1547,7 @@ struct sloth_shading {
     uint64_t n_bwd_tiles = 0;
     DevBuf d_dcolor, d_dnormal;
     std::vector<double> partial;
+    uint32_t n_partials = 0;   // per-block partial sums the last pass wrote
 };
 
 static ShadeArgs shade_args(sloth_shading* sh, bool with_target, float* out)
@@ -1705,7 +1706,7 @@ int sloth_shading_set_target(sloth_shading* sh, const float* target)
 
 static int shading_reduce(sloth_shading* sh, double* loss, double* dlight)
 {
-    const uint32_t nb = shade_blocks(sh->n_slots);
+    const uint32_t nb = sh->n_partials;
     if (nb) HIP_TRY(hipMemcpy(sh->partial.data(), sh->d_partial.p, (size_t)nb * 32, hipMemcpyDeviceToHost));
     double acc[4] = {0, 0, 0, 0};
     for (uint32_t i = 0; i < nb; ++i)
@@ -1736,6 +1737,7 @@ static int shading_backward_launch(sloth_shading* sh)
     const ShadeArgs a = shade_args(sh, true, nullptr);
     int rc = launch_shade(a, true, s);
     if (rc) return rc;
+    sh->n_partials = bwd_blocks(sh->n_slots, sh->n_bwd_tiles);
     return launch_shade_finish(a, sh->d_dcolor.as<float>(), sh->d_dnormal.as<float>(), s);
 }
 
@@ -1771,6 +1773,7 @@ int sloth_shading_forward(sloth_shading* sh, const float* vcolor, const float* v
         HIP_TRY(hipMemsetAsync(sh->d_out.p, 0, sh->n_slots * 12, s));
     }
     if ((rc = launch_shade(shade_args(sh, sh->has_target, d_out), false, s))) return rc;
+    sh->n_partials = shade_blocks(sh->n_slots);
     if (out_rgb) HIP_TRY(hipMemcpyAsync(out_rgb, sh->d_out.p, sh->n_slots * 12, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return shading_reduce(sh, loss, nullptr);

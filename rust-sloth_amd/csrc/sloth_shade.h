@@ -30,6 +30,8 @@ struct ShadeArgs {
 };
 
 uint32_t shade_blocks(uint64_t n_slots);
+// workgroups (and per-block partial sums) of the backward pass
+uint32_t bwd_blocks(uint64_t n_slots, uint64_t n_bwd_tiles);
 // Backward tiles per frame, prefix-summed: begin[0..n_frames]
 void bwd_tiles(const Frame* frames, uint32_t n_frames, uint64_t* begin);
 int launch_shade(const ShadeArgs& a, bool backward, hipStream_t s);

@@ -297,6 +297,14 @@ __global__ void k_shade_finish(const double* __restrict__ vert_acc, uint32_t n_v
 
 uint32_t shade_blocks(uint64_t n_slots) { return (uint32_t)((n_slots + SHADE_BLOCK - 1) / SHADE_BLOCK); }
 
+// the backward's workgroups: one per tile, at most BWD_MAX_BLOCKS (which then
+// stride over the tiles), never more than the forward's partial sums
+constexpr uint32_t BWD_MAX_BLOCKS = 8192;
+uint32_t bwd_blocks(uint64_t n_slots, uint64_t n_bwd_tiles)
+{
+    return (uint32_t)std::min<uint64_t>({n_bwd_tiles, BWD_MAX_BLOCKS, shade_blocks(n_slots)});
+}
+
 void bwd_tiles(const Frame* frames, uint32_t n_frames, uint64_t* begin)
 {
     begin[0] = 0;
@@ -315,9 +323,8 @@ int launch_shade(const ShadeArgs& a, bool backward, hipStream_t s)
         hipLaunchKernelGGL(k_fill_f64, dim3(std::max(1u, std::min(4096u, (uint32_t)((n_acc + 255) / 256)))),
                            dim3(256), 0, s, a.vert_acc, n_acc);
         HIP_TRY(hipGetLastError());
-        // nb blocks (the partial sums' count) stride over the backward tiles
-        if (a.n_bwd_tiles) hipLaunchKernelGGL(k_shade_bwd, dim3(nb), dim3(SHADE_BLOCK), 0, s, a, a.bwd_tile_begin,
-                                              a.n_bwd_tiles);
+        const uint32_t g = bwd_blocks(a.n_slots, a.n_bwd_tiles);
+        if (g) hipLaunchKernelGGL(k_shade_bwd, dim3(g), dim3(SHADE_BLOCK), 0, s, a, a.bwd_tile_begin, a.n_bwd_tiles);
     } else {
         hipLaunchKernelGGL(k_shade_fwd, dim3(nb), dim3(SHADE_BLOCK), 0, s, a);
     }
