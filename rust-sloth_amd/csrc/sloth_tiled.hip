@@ -260,25 +260,40 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         // a closed mesh's, in runs) has no tiles to find: a uniform branch.
         if (__ballot(boxed)) {
             int bx0 = 1, bx1 = 0;
-            if (boxed) box_x(t, cls == CLS_TIGHT, bx0, bx1);
-            TileRects r;
-            tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);   // empty when bx0 > bx1
+            if (boxed) box_x(t, cls == CLS_TIGHT, bx0, bx1);   // empty when not boxed
             if (COUNT) {
                 const bool big = boxed && (int64_t)(bx1 - bx0 + 1) * (int64_t)(t.y1d - t.y0) > (int64_t)BIG_BOX;
                 const uint64_t m = __ballot(big);
                 if (__lane_id() == 0 && m) atomicAdd(&careful_cnt[BIG_CNT], (uint32_t)__popcll(m));
             }
-            const bool has_direct = r.dx0 <= r.dx1 && r.dy0 <= r.dy1;
-            const bool has_wrap = r.wx0 <= r.wx1 && r.wy0 <= r.wy1;
             // A box in one region appends through that region's tile
             // rectangle [qx0, qx1] x [qy0, qy1]; a box that wraps in part
             // (it spans 2x = W) takes the per-tile path.  (Meshes wider than
             // the frame's logical width put whole patches of triangles in
             // the wrapped region, whose per-tile atomics would meet on few
             // addresses.)
-            const bool wrap_only = has_wrap && !has_direct;
-            const int qx0 = wrap_only ? r.wx0 : r.dx0, qx1 = wrap_only ? r.wx1 : r.dx1;
-            const int qy0 = wrap_only ? r.wy0 : r.dy0, qy1 = wrap_only ? r.wy1 : r.dy1;
+            bool has_direct, has_wrap;
+            int qx0, qx1, qy0, qy1;
+            if (!__ballot(bx1 >= (int)((W + 1) >> 1))) {
+                // no box of the wavefront reaches the wrapped region (x >=
+                // (W + 1) / 2): tile_rects' direct rectangle with x <= bx1
+                has_direct = bx0 <= bx1 && t.y0 < t.y1d;
+                has_wrap = false;
+                qx0 = (2 * bx0) >> tws;
+                qx1 = (2 * bx1) >> tws;
+                qy0 = t.y0 >> ths;
+                qy1 = (t.y1d - 1) >> ths;
+            } else {
+                TileRects r;
+                tile_rects(bx0, bx1, t.y0, t.y1d - 1, W, tws, ths, r);   // empty when bx0 > bx1
+                has_direct = r.dx0 <= r.dx1 && r.dy0 <= r.dy1;
+                has_wrap = r.wx0 <= r.wx1 && r.wy0 <= r.wy1;
+                const bool wrap_only = has_wrap && !has_direct;
+                qx0 = wrap_only ? r.wx0 : r.dx0;
+                qx1 = wrap_only ? r.wx1 : r.dx1;
+                qy0 = wrap_only ? r.wy0 : r.dy0;
+                qy1 = wrap_only ? r.wy1 : r.dy1;
+            }
             const bool sm = has_direct != has_wrap && qx1 - qx0 <= 1 && qy1 - qy0 <= 1;
             if (has_direct || has_wrap) tiled |= 1u << k;
             if (sm) {
