@@ -137,3 +137,38 @@ def test_uneven_batch_autotuned(sloth):
             cells, z, _ = b.read_frame(i)
             ref = oracle.render(W, H, U, m.xyz, m.rgb, R)
             assert_frame_equal(cells, z, None, ref, W)
+
+
+def test_frame_past_the_tile_key(sloth):
+    """A frame of 4 x 1,310,720 = 5,242,880 triangles (more than the tiled
+    path's 2^22 - 1 per frame) takes the general global-key path by itself
+    (the plan has no tiles), beside a small frame in the same batch, and
+    both match the oracle."""
+    from sloth_amd import meshgen
+
+    m, _, _, _, R = meshgen.workload_icosphere_4k()
+    small = meshgen.uv_sphere(16, 8, seed=50)
+    W, H = 160, 48
+    U = meshgen.screen_transform(W, H, [m.bbox_max()])
+    U2 = meshgen.screen_transform(64, 30, [small.bbox_max()])
+    Rs = [meshgen.rotation_euler(0.3 * k, 0.7 * k, 0.1) for k in range(4)]
+    sm, ss = sloth.SimpleMesh.from_mesh(m), sloth.SimpleMesh.from_mesh(small)
+    say(f"case frame of {4 * m.n} triangles at {W}x{H} beside a small frame")
+    b = sloth.Batch([sloth.FrameSpec(W, H, U, True), sloth.FrameSpec(64, 30, U2, True)],
+                    [sloth.DrawSpec(0, sm, R_) for R_ in Rs] + [sloth.DrawSpec(1, ss, R)])
+    b.run()
+    b.sync()
+    assert b.plan()["tile_w"] == 0
+    ref = oracle_render_draws(W, H, U, [(m.xyz, m.rgb, R_) for R_ in Rs])
+    cells, z, win = b.read_frame(0, want_winners=True)
+    assert_frame_equal(cells, z, win, ref, W)
+    ref2 = oracle_render_draws(64, 30, U2, [(small.xyz, small.rgb, R)])
+    cells, z, win = b.read_frame(1, want_winners=True)
+    assert_frame_equal(cells, z, win, ref2, 64)
+
+
+def oracle_render_draws(W, H, U, draws):
+    f = oracle.Frame(W, H, True)
+    for xyz, rgb, R in draws:
+        f.draw_mesh(U, xyz, rgb, R)
+    return f
