@@ -8,12 +8,13 @@
 //
 //   k_tile_clear   tile cursors <- 0, newline rows <- -1, overflow <- 0,
 //                  careful-list length <- 0
-//   k_bin          one thread per triangle: setup, classification, candidate
-//                  box, newline rows (LDS max per block, one atomicMax per
-//                  row), and the triangle index appended to every tile its box
-//                  reaches: boxes within 2 x 2 tiles count into a 16 x 16 LDS
-//                  window of tiles, with one global atomicAdd per window tile
-//                  per block; others (and the careful list) append directly
+//   k_bin          1, 2 or 4 triangles per thread: setup, classification,
+//                  candidate box, newline rows (LDS max per block, one
+//                  atomicMax per row), and the triangle index appended to
+//                  every tile its box reaches: boxes within 2 x 2 tiles of one
+//                  region count into a 16 x 16 LDS window of tiles, with one
+//                  global atomicAdd per window tile per block; others (and the
+//                  careful list) append directly
 //   k_careful      one wavefront per careful triangle: exact box from row
 //                  spans, then its tile entries
 //   k_tile_raster  one 256-thread workgroup per tile, grid (tile x, tile y,

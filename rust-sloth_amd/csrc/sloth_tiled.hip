@@ -184,16 +184,17 @@ __device__ __forceinline__ int wave_max(int v)
 
 // Grid: x = blocks of TPT * BIN_BLOCK consecutive triangles of one draw,
 // y = draw index - d_lo, so the draw, its frame and its matrix are uniform
-// per block (scalar loads).  CLS_BOX triangles are appended to their tiles
-// here; CLS_SPANS / CLS_FULL ones go to the careful list for k_careful.
+// per block (scalar loads).  CLS_BOX / CLS_TIGHT triangles are appended to
+// their tiles here; CLS_SPANS / CLS_FULL ones go to the careful list for
+// k_careful.
 //
 // Each thread takes TPT triangles (k * BIN_BLOCK + thread), set up one after
 // another with their loads in flight together, so that the block's barriers
 // and its global atomic round trip serve TPT times as many triangles (the
 // kernel is bound by latency, not by issue).  TPT is 1, 2 or 4
-// (TiledPlan::bin_tpt; the autotuner picks it).  Per triangle a
-// thread keeps two words between the phases: its tile rectangle and flags,
-// and its rows.
+// (TiledPlan::bin_tpt; the autotuner picks it).  Per triangle a thread
+// keeps two words between the phases (its tile rectangle and flags, and its
+// rows) and two more of window counts between the last two.
 //
 // Appends: a block's consecutive triangles lie close together on screen.
 // Triangles whose box touches at most 2 x 2 tiles, all in the direct region
@@ -203,6 +204,7 @@ __device__ __forceinline__ int wave_max(int v)
 // thread that reserves it also reads the tile's pool range), and each
 // triangle writes its entry at run + its LDS count.  Others append with one
 // global atomicAdd per tile.
+//
 // geo word: qx0 (bits 0-13), qy0 (14-27) of the tile rectangle, wide (28),
 // tall (29), CLS_BOX (30); tile coordinates stay below 2^14 (frames of at
 // most 65535 cells a side, tiles of at least 8 x 4 cells).
