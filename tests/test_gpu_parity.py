@@ -429,6 +429,7 @@ def test_autotune_keeps_results(sloth, raster_path):
     say(f"case autotune -> {tw}x{th}")
     assert tw in (16, 32, 64, 128, 256, 512) and th in (8, 16, 32, 64)
     assert b.plan()["tile_w"] == tw and b.plan()["tile_h"] == th
+    assert b.plan()["bin_tris_per_thread"] in (1, 2, 4)
     b.run()
     b.sync()
     for k, R in enumerate(Rs):
