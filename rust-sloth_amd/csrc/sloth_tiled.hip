@@ -808,11 +808,64 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                                         (uint64_t*)(win + fr.z_off + id));
         }
     };
+    // Even widths, final stores: a thread takes the slot pairs (2q, 2q + 1) of
+    // a row, q = thread mod (slots a row / 2), every (512 >> sws)-th row, and
+    // writes a pair's 4 cells, 2 depth pairs and winners with one 16-byte
+    // store each where the address is 16-byte aligned (two 8-byte ones
+    // otherwise).  fn(first slot, row, id of its first cell, that cell's
+    // column, the second slot lies in the frame).
+    auto for_my_pairs = [&](auto&& fn) {
+        const uint32_t q = threadIdx.x & ((1u << (sws - 1)) - 1);
+        const int col = g.tc0 + ((int)q << 2);
+        if (col >= g.tc1) return;
+        const int step = (int)(blockDim.x >> (sws - 1));
+        int row = g.tr0 + (int)(threadIdx.x >> (sws - 1));
+        uint64_t id = (uint64_t)row * W + (uint64_t)col;
+        for (; row < g.tr1; row += step, id += (uint64_t)step * W)
+            fn(((uint32_t)(row - g.tr0) << sws) | (q << 1), row, id, col, col + 2 < g.tc1);
+    };
+    auto put_pair = [](uint32_t* p, uint32_t a, uint32_t b, uint32_t c, uint32_t d, bool first, bool second) {
+        typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+        if (first && second && ((uintptr_t)p & 15u) == 0) {
+            const v4u v = {a, b, c, d};
+            __builtin_nontemporal_store(v, (v4u*)p);
+            return;
+        }
+        if (first) __builtin_nontemporal_store((uint64_t)a | ((uint64_t)b << 32), (uint64_t*)p);
+        if (second) __builtin_nontemporal_store((uint64_t)c | ((uint64_t)d << 32), (uint64_t*)(p + 2));
+    };
+    // Stores the slots of a pair that `need0` / `need1` name, as store()
+    // stores one slot (winning cell or blank, depth or f32::MAX, winner).
+    auto store_pair = [&](int row, uint64_t id, int col, uint64_t k0, uint32_t c0, float z0, bool need0, uint64_t k1,
+                          uint32_t c1, float z1, bool need1) {
+        const int32_t t0 = (k0 == KEY_EMPTY) ? -1 : (int32_t)tile_key_tri(k0);
+        const int32_t t1 = (k1 == KEY_EMPTY) ? -1 : (int32_t)tile_key_tri(k1);
+        uint32_t c0b = c0;
+        if (fr.image && col == 0) {
+            // cell row*W + 1, as in store()
+            const int32_t nlr = fnl[row];
+            if (nlr >= 0 && (k0 == KEY_EMPTY || nlr >= t0)) c0b = CELL_NEWLINE;
+        }
+        put_pair(fc + id, c0, c0b, c1, c1, need0, need1);
+        put_pair((uint32_t*)(fz + id), __float_as_uint(z0), __float_as_uint(F32_MAX), __float_as_uint(z1),
+                 __float_as_uint(F32_MAX), need0, need1);
+        if (win) {
+            auto wv = [&](uint64_t k, int32_t t) -> uint32_t {
+                return (uint32_t)((k == KEY_EMPTY) ? -1 : (win_raw ? (int32_t)((uint32_t)t << 1 | tile_key_sub(k)) : t));
+            };
+            put_pair((uint32_t*)(win + fr.z_off + id), wv(k0, t0), 0xFFFFFFFFu, wv(k1, t1), 0xFFFFFFFFu, need0, need1);
+        }
+    };
 
     if (n == 0) {
         // no triangle reaches the tile: blank cells, f32::MAX depths, and the
         // newline cells of column 1
-        for_my_slots([&](uint32_t s, int row, uint64_t id) { store(s, row, id, KEY_EMPTY, CELL_BLANK, F32_MAX); });
+        if constexpr (ODD)
+            for_my_slots([&](uint32_t s, int row, uint64_t id) { store(s, row, id, KEY_EMPTY, CELL_BLANK, F32_MAX); });
+        else
+            for_my_pairs([&](uint32_t, int row, uint64_t id, int col, bool two) {
+                store_pair(row, id, col, KEY_EMPTY, CELL_BLANK, F32_MAX, true, KEY_EMPTY, CELL_BLANK, F32_MAX, two);
+            });
     } else {
         // glyph of shade sh under shader r: the first LDS_RAMPS shaders from
         // LDS, the others from global memory (two calls, so that each reads
@@ -875,7 +928,7 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         // 0xFF); the depth then comes from the key's code, except for a
         // zero depth, whose sign the code does not keep.  Other winners are
         // re-evaluated from the transformed triangle.
-        auto winner = [&](int row, uint64_t k, uint32_t& cell, float& z) {
+        auto winner = [&](int col, int row, uint64_t k, uint32_t& cell, float& z) {
             const uint32_t sub = tile_key_sub(k);
             const float4* row4 = (const float4*)(cache + tile_key_row(k) * CACHE_WORDS);
             const float4 q2 = row4[2];
@@ -888,7 +941,6 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                 return;
             }
             const float4 q0 = row4[0], q1 = row4[1];
-            const int col = my_col;
             const float px = (float)(sub ? (col >> 1) : ((col + g.W) >> 1));
             const float py = (float)(sub ? row : row - 1);
             // the same expressions as edges(), depth() and shade_value()
@@ -1032,19 +1084,33 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
                     if (k == KEY_EMPTY || (k & TILE_KEY_STORED)) return;
                     uint32_t cell;
                     float z;
-                    winner(row, k, cell, z);
+                    winner(my_col, row, k, cell, z);
                     store(s, row, id, k, cell, z);
                     keys[s] = k | TILE_KEY_STORED;
                 });
         }
-        for_my_slots([&](uint32_t s, int row, uint64_t id) {
-            const uint64_t k = keys[s];
-            if (multi && k != KEY_EMPTY) return;   // stored after its chunk
-            uint32_t cell = CELL_BLANK;
-            float z = F32_MAX;
-            if (k != KEY_EMPTY) winner(row, k, cell, z);
-            store(s, row, id, k, cell, z);
-        });
+        if constexpr (ODD) {
+            for_my_slots([&](uint32_t s, int row, uint64_t id) {
+                const uint64_t k = keys[s];
+                if (multi && k != KEY_EMPTY) return;   // stored after its chunk
+                uint32_t cell = CELL_BLANK;
+                float z = F32_MAX;
+                if (k != KEY_EMPTY) winner(my_col, row, k, cell, z);
+                store(s, row, id, k, cell, z);
+            });
+        } else {
+            for_my_pairs([&](uint32_t s, int row, uint64_t id, int col, bool two) {
+                const uint64_t k0 = keys[s], k1 = two ? keys[s + 1] : KEY_EMPTY;
+                const bool need0 = !(multi && k0 != KEY_EMPTY);   // else stored after its chunk
+                const bool need1 = two && !(multi && k1 != KEY_EMPTY);
+                if (!need0 && !need1) return;
+                uint32_t c0 = CELL_BLANK, c1 = CELL_BLANK;
+                float z0 = F32_MAX, z1 = F32_MAX;
+                if (need0 && k0 != KEY_EMPTY) winner(col, row, k0, c0, z0);
+                if (need1 && k1 != KEY_EMPTY) winner(col + 2, row, k1, c1, z1);
+                store_pair(row, id, col, k0, c0, z0, need0, k1, c1, z1, need1);
+            });
+        }
     }
     if constexpr (ODD) {
         // Each cell: the later write of its own slot's winner and the left
