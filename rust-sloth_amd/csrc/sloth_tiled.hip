@@ -155,7 +155,7 @@ __device__ __forceinline__ void put_entry(uint64_t gt, uint32_t pos, uint32_t en
 // ---------------------------------------------------------------------------
 // k_bin
 // ---------------------------------------------------------------------------
-constexpr int BIN_WIN = 16;
+constexpr int BIN_WIN = 16;   // the block aggregates appends in a 16 x 16 window of tiles
 
 // Wavefront-wide min / max (all 64 lanes must be active), returned uniform.
 // A block reduces through these first so that each wavefront issues one LDS
@@ -180,7 +180,7 @@ __device__ __forceinline__ int wave_min(int v)
 __device__ __forceinline__ int wave_max(int v)
 {
     return wave_reduce(v, [](int a, int b) { return max(a, b); });
-}   // the block aggregates appends in a 16 x 16 window of tiles
+}
 
 // Grid: x = blocks of TPT * BIN_BLOCK consecutive triangles of one draw,
 // y = draw index - d_lo, so the draw, its frame and its matrix are uniform
