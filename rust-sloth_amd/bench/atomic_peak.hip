@@ -9,6 +9,11 @@
 //   global_add_f64  binary64 global atomicAdd without return, pseudo-random
 //                   entries of a 1.2 MB table (the shading gradients of a
 //                   50,000-vertex mesh)
+//   lds_add_f32     binary32 LDS atomicAdd without return (ds_add_f32),
+//                   pseudo-random slots of a 1024-slot table
+//   lds_add_u64     64-bit integer LDS atomicAdd without return (ds_add_u64),
+//                   the same pattern (the shading backward's fixed-point
+//                   vertex sums)
 //
 // Prints one JSON object: operations per second for each (best of 5 runs),
 // and the grid.  Every address is masked into its table, so no access leaves
@@ -43,6 +48,25 @@ __global__ void __launch_bounds__(BLOCK) k_lds_min(uint64_t* out, int iters)
         uint64_t m = ~0ull;
         for (int i = 0; i < LDS_SLOTS; ++i) m = m < keys[i] ? m : keys[i];
         out[blockIdx.x] = m;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_lds_add(uint64_t* out, int iters)
+{
+    __shared__ T sums[LDS_SLOTS];
+    for (int i = threadIdx.x; i < LDS_SLOTS; i += BLOCK) sums[i] = 0;
+    __syncthreads();
+    uint32_t h = (blockIdx.x * BLOCK + threadIdx.x) * 2654435761u + 4242u;
+    for (int k = 0; k < iters; ++k) {
+        h = step(h);
+        atomicAdd(&sums[h >> 22], (T)(h & 7u));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T m = 0;
+        for (int i = 0; i < LDS_SLOTS; ++i) m += sums[i];
+        out[blockIdx.x] = (uint64_t)m;
     }
 }
 
@@ -90,15 +114,18 @@ int main()
     hipEvent_t a, b;
     CHECK(hipEventCreate(&a));
     CHECK(hipEventCreate(&b));
-    double rate[3];
-    const int iters[3] = {4096, 256, 64};
-    for (int which = 0; which < 3; ++which) {
+    double rate[5];
+    const int iters[5] = {4096, 256, 64, 512, 4096};
+    for (int which = 0; which < 5; ++which) {
         float best = 1e30f;
         for (int rep = 0; rep < 6; ++rep) {   // the first run warms up
             CHECK(hipEventRecord(a, 0));
             if (which == 0) hipLaunchKernelGGL(k_lds_min, dim3(grid), dim3(BLOCK), 0, 0, out, iters[0]);
             if (which == 1) hipLaunchKernelGGL(k_global_min, dim3(grid), dim3(BLOCK), 0, 0, table, iters[1]);
             if (which == 2) hipLaunchKernelGGL(k_global_add, dim3(grid), dim3(BLOCK), 0, 0, acc, iters[2]);
+            if (which == 3) hipLaunchKernelGGL(k_lds_add<float>, dim3(grid), dim3(BLOCK), 0, 0, out, iters[3]);
+            if (which == 4)
+                hipLaunchKernelGGL(k_lds_add<unsigned long long>, dim3(grid), dim3(BLOCK), 0, 0, out, iters[4]);
             CHECK(hipGetLastError());
             CHECK(hipEventRecord(b, 0));
             CHECK(hipEventSynchronize(b));
@@ -109,8 +136,9 @@ int main()
         rate[which] = (double)grid * BLOCK * iters[which] / (best * 1e-3);
     }
     printf("{\"lds_min_u64_per_s\": %.4e, \"global_min_u64_per_s\": %.4e, \"global_add_f64_per_s\": %.4e, "
+           "\"lds_add_f32_per_s\": %.4e, \"lds_add_u64_per_s\": %.4e, "
            "\"grid\": %d, \"block\": %d, \"cus\": %d}\n",
-           rate[0], rate[1], rate[2], grid, BLOCK, p.multiProcessorCount);
+           rate[0], rate[1], rate[2], rate[3], rate[4], grid, BLOCK, p.multiProcessorCount);
     CHECK(hipFree(out));
     CHECK(hipFree(table));
     CHECK(hipFree(acc));

@@ -35,51 +35,96 @@ __device__ __forceinline__ const Draw& shade_draw(const Draw* __restrict__ draws
     return draws[lo];
 }
 
-// One visible fragment of z slot g (frame fr, visibility word v >= 0): the
-// barycentric weights, the interpolated normal (mapped by G) and colour, and
-// the output colour.
-struct Fragment {
-    float lam[3];
-    float n0, n1, n2, c0, c1, c2, I;
-    const float* G;
-    uint32_t vid[3];   // the triangle's vertices
+// What every fragment of triangle T (frame fr) uses: its screen vertices,
+// edge deltas and 1/orient as draw_triangle computes them (rasterizer.rs:56-67),
+// its vertex ids, the normal map G applied to each vertex normal, the vertex
+// colours, and G^T L (the light in model space for this view).
+struct ShadeTri {
+    float x1, y1, x2, y2, x3, y3, A0, B0, A1, B1, A2, B2, a;
+    float gn[3][3];    // G n[v_k]
+    float col[3][3];   // c[v_k]
+    float gl[3];       // G^T L
+    uint32_t vid[3];
 };
 
-__device__ __forceinline__ Fragment shade_fragment(const ShadeArgs& p, const Frame& fr, uint64_t g, int32_t v)
+__device__ __forceinline__ void shade_tri(const ShadeArgs& p, const Frame& fr, uint32_t T, ShadeTri& s)
 {
-    Fragment f;
-    const uint32_t T = (uint32_t)v >> 1, sub = (uint32_t)v & 1u;
-    uint32_t px, py;
-    key_pixel((uint32_t)(g - fr.z_off), sub, fr.W, px, py);
     const uint32_t d = (uint32_t)(&shade_draw(p.draws, fr, T) - p.draws);
     const Draw& dr = p.draws[d];
     const uint32_t li = T - dr.tri_begin;
     Tri t;
     setup_draw_tri(dr, li, fr.W, fr.H, t);
-    float w0, w1, w2;
-    edges(t, (float)px, (float)py, w0, w1, w2);
-    f.lam[0] = t.a * w0;
-    f.lam[1] = t.a * w1;
-    f.lam[2] = t.a * w2;
-    f.G = p.G + (size_t)d * 9;
-    const float* G = f.G;
-    f.n0 = f.n1 = f.n2 = f.c0 = f.c1 = f.c2 = 0.f;
+    s.x1 = t.x1;
+    s.y1 = t.y1;
+    s.x2 = t.x2;
+    s.y2 = t.y2;
+    s.x3 = t.x3;
+    s.y3 = t.y3;
+    s.A0 = t.A0;
+    s.B0 = t.B0;
+    s.A1 = t.A1;
+    s.B1 = t.B1;
+    s.A2 = t.A2;
+    s.B2 = t.B2;
+    s.a = t.a;
+    const float* G = p.G + (size_t)d * 9;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const uint32_t vid = p.faces[(size_t)li * 3 + k];
-        f.vid[k] = vid;
+        s.vid[k] = vid;
         const float* nk = p.vnormal + (size_t)vid * 3;
         const float* ck = p.vcolor + (size_t)vid * 3;
         const float a0 = nk[0], a1 = nk[1], a2 = nk[2];
-        f.n0 += f.lam[k] * (G[0] * a0 + G[1] * a1 + G[2] * a2);
-        f.n1 += f.lam[k] * (G[3] * a0 + G[4] * a1 + G[5] * a2);
-        f.n2 += f.lam[k] * (G[6] * a0 + G[7] * a1 + G[8] * a2);
-        f.c0 += f.lam[k] * ck[0];
-        f.c1 += f.lam[k] * ck[1];
-        f.c2 += f.lam[k] * ck[2];
+        s.gn[k][0] = G[0] * a0 + G[1] * a1 + G[2] * a2;
+        s.gn[k][1] = G[3] * a0 + G[4] * a1 + G[5] * a2;
+        s.gn[k][2] = G[6] * a0 + G[7] * a1 + G[8] * a2;
+        s.col[k][0] = ck[0];
+        s.col[k][1] = ck[1];
+        s.col[k][2] = ck[2];
+    }
+    const float L0 = p.light[0], L1 = p.light[1], L2 = p.light[2];
+    s.gl[0] = G[0] * L0 + G[3] * L1 + G[6] * L2;
+    s.gl[1] = G[1] * L0 + G[4] * L1 + G[7] * L2;
+    s.gl[2] = G[2] * L0 + G[5] * L1 + G[8] * L2;
+}
+
+// One visible fragment of triangle s at pixel (px, py): the barycentric
+// weights, the interpolated normal (mapped by G) and colour, and the shade I.
+struct Fragment {
+    float lam[3];
+    float n0, n1, n2, c0, c1, c2, I;
+};
+
+__device__ __forceinline__ Fragment shade_at(const ShadeArgs& p, const ShadeTri& s, float px, float py)
+{
+    Fragment f;
+    // the expressions of edges() (rasterizer.rs:71-74)
+    const float w0 = s.A0 * (py - s.y2) - s.B0 * (px - s.x2);
+    const float w1 = s.A1 * (py - s.y3) - s.B1 * (px - s.x3);
+    const float w2 = s.A2 * (py - s.y1) - s.B2 * (px - s.x1);
+    f.lam[0] = s.a * w0;
+    f.lam[1] = s.a * w1;
+    f.lam[2] = s.a * w2;
+    f.n0 = f.n1 = f.n2 = f.c0 = f.c1 = f.c2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        f.n0 += f.lam[k] * s.gn[k][0];
+        f.n1 += f.lam[k] * s.gn[k][1];
+        f.n2 += f.lam[k] * s.gn[k][2];
+        f.c0 += f.lam[k] * s.col[k][0];
+        f.c1 += f.lam[k] * s.col[k][1];
+        f.c2 += f.lam[k] * s.col[k][2];
     }
     f.I = p.light[0] * f.n0 + p.light[1] * f.n1 + p.light[2] * f.n2;
     return f;
+}
+
+// The pixel of z entry (row, col) of a frame of width W whose winner has
+// `sub` (key_pixel without the division).
+__device__ __forceinline__ void entry_pixel(uint32_t row, uint32_t col, uint32_t sub, uint32_t W, float& px, float& py)
+{
+    px = (float)(sub ? col >> 1 : (col + W) >> 1);
+    py = (float)(sub ? row : row - 1);
 }
 
 // Block sums of up to four doubles in a fixed order, written to partial[block * 4 + k].
@@ -121,7 +166,13 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_fwd(ShadeArgs p)
         }
         const int32_t v = p.vis[g];
         if (v >= 0) {
-            const Fragment f = shade_fragment(p, p.frames[lo], g, v);
+            const Frame& fr = p.frames[lo];
+            const uint32_t id = (uint32_t)(g - fr.z_off), row = id / fr.W;
+            ShadeTri st;
+            shade_tri(p, fr, (uint32_t)v >> 1, st);
+            float px, py;
+            entry_pixel(row, id - row * fr.W, (uint32_t)v & 1u, fr.W, px, py);
+            const Fragment f = shade_at(p, st, px, py);
             const float o0 = f.I * f.c0, o1 = f.I * f.c1, o2 = f.I * f.c2;
             if (p.out) {
                 p.out[g * 3 + 0] = o0;
@@ -141,31 +192,62 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_fwd(ShadeArgs p)
 
 constexpr uint32_t BWD_TILE_W = 64, BWD_TILE_H = 32;   // z entries per backward tile
 constexpr uint32_t BWD_TABLE = 256;     // LDS rows for the tile's distinct vertices
-constexpr uint32_t BWD_PROBES = 32;     // a fragment that finds no row adds to HBM directly
-constexpr uint32_t BWD_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t BWD_PROBES = 32;     // a vertex that finds no row adds to HBM directly
+constexpr uint64_t BWD_EMPTY = ~0ull;
 constexpr uint32_t BWD_RUN = 8;         // consecutive z entries of a row per thread
+constexpr uint32_t BWD_GROUP = 4;       // run entries whose loads are issued together
+constexpr uint32_t BWD_ROW = 7;         // 64-bit LDS words per vertex row (6 terms, padded)
 static_assert(BWD_TILE_W % BWD_RUN == 0 && SHADE_BLOCK / (BWD_TILE_W / BWD_RUN) == BWD_TILE_H,
               "the block's runs cover one tile in one pass");
+static_assert(BWD_TABLE == SHADE_BLOCK, "one thread per table row at the end of a tile");
+
+// Vertex rows hold their 6 sums in fixed point: term t is added as the
+// integer round(t 2^(FIX_BITS - e)), with e fixed per row when the row is
+// claimed (the claiming sums' largest binary exponent + FIX_HEADROOM), so
+// that |integer| < 2^FIX_BITS for every |t| < 2^e and 2^(63 - FIX_BITS)
+// adds cannot overflow (a row takes at most one add per z entry of the
+// tile, 2048).  A term outside the row's range (or not finite) goes to the
+// binary64 accumulator directly.  The rounding error of an add is at most
+// 2^(e - FIX_BITS - 1), i.e. 2^-39 of the claiming sums' magnitude.  LDS has
+// integer atomics at full rate, while a float atomic add takes several
+// cycles per lane.
+constexpr int FIX_BITS = 50;
+constexpr int FIX_HEADROOM = 12;
+static_assert(BWD_TILE_W * BWD_TILE_H <= (1u << (63 - FIX_BITS)), "a row's adds stay within int64");
+
+__device__ __forceinline__ int fix_exponent(const float* t)
+{
+    int e = -100;
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+        if (t[m] != 0.0f) e = max(e, __builtin_amdgcn_frexp_expf(t[m]));
+    return min(e, 100) + FIX_HEADROOM;
+}
 
 // Backward: a block takes 64 x BWD_TILE_H tiles of z entries (tile j of the
 // job; frames hold tile_begin[f] .. tile_begin[f+1]), grid-striding over the
-// tiles, each thread a run of BWD_RUN entries of a row.  A fragment's 6
-// colour / normal terms for each of its three vertices are summed over the
-// run's consecutive fragments of one triangle, then go into an LDS table
-// keyed by vertex; after the tile, each table row adds its sums to the
-// vertex's binary64 accumulators with one global atomic per term.  Several pixels of a tile share each vertex, so this takes most
+// tiles, each thread a run of BWD_RUN entries of a row (every other entry
+// for even widths, whose odd entries hold no winner).  The run's visibility
+// words, then the targets of its fragments, are loaded BWD_GROUP entries at
+// a time with their loads in flight together.  A fragment's 6 colour /
+// normal terms for each of its three vertices are summed over the run's
+// consecutive fragments of one triangle, then go into an LDS table keyed by
+// vertex, in fixed point (above); after the tile, each table row adds its
+// sums to the vertex's binary64 accumulators with one global atomic per
+// term.  Several pixels of a tile share each vertex, so this takes most
 // same-address atomics off the memory system.  A vertex that finds no free
 // row adds its terms to the accumulators directly.  The loss and the light
 // gradient are summed per block.
 __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_bwd(ShadeArgs p, const uint64_t* __restrict__ tile_begin,
                                                            uint64_t n_tiles)
 {
-    __shared__ uint32_t s_key[BWD_TABLE];
-    __shared__ float s_acc[BWD_TABLE * 6];
+    // key: exponent << 32 | vertex id (set together by the claiming CAS)
+    __shared__ uint64_t s_key[BWD_TABLE];
+    __shared__ uint64_t s_acc[BWD_TABLE * BWD_ROW];
     double loss = 0.0, dl0 = 0.0, dl1 = 0.0, dl2 = 0.0;
     for (uint64_t j = blockIdx.x; j < n_tiles; j += gridDim.x) {
-        for (uint32_t i = threadIdx.x; i < BWD_TABLE; i += SHADE_BLOCK) s_key[i] = BWD_EMPTY;
-        for (uint32_t i = threadIdx.x; i < BWD_TABLE * 6; i += SHADE_BLOCK) s_acc[i] = 0.0f;
+        s_key[threadIdx.x] = BWD_EMPTY;
+        for (uint32_t i = threadIdx.x; i < BWD_TABLE * BWD_ROW; i += SHADE_BLOCK) s_acc[i] = 0;
         __syncthreads();
         uint32_t lo = 0, hi = p.n_frames - 1;
         while (lo < hi) {
@@ -173,101 +255,132 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_bwd(ShadeArgs p, const ui
             if (tile_begin[mid] <= j) lo = mid; else hi = mid - 1;
         }
         const Frame& fr = p.frames[lo];
-        const uint32_t tiles_x = (fr.W + BWD_TILE_W - 1) / BWD_TILE_W;
+        const uint32_t W = fr.W;
+        const uint32_t tiles_x = (W + BWD_TILE_W - 1) / BWD_TILE_W;
         const uint32_t local = (uint32_t)(j - tile_begin[lo]);
         const uint32_t ty = local / tiles_x, tx = local - ty * tiles_x;
         // Each thread takes BWD_RUN consecutive z entries of a row.  Their
         // terms are summed in registers while they share a triangle and go
         // to the table when the triangle changes or the run ends: adjacent
-        // pixels mostly share a triangle, and lanes that add to the same
-        // LDS row in one instruction are serialised.
+        // pixels mostly share a triangle.
         constexpr uint32_t RUNS = BWD_TILE_W / BWD_RUN;   // threads per tile row
         const uint32_t col0 = tx * BWD_TILE_W + (threadIdx.x % RUNS) * BWD_RUN;
-        const float L0 = p.light[0], L1 = p.light[1], L2 = p.light[2];
-        int32_t cur = -1;   // triangle of the sums below, or none
-        uint32_t cvid[3] = {0, 0, 0};
+        const uint32_t cstep = (W & 1u) ? 1u : 2u;        // runs start on even columns
+        int32_t cur = -1;   // triangle of the sums below and of st, or none
+        ShadeTri st;
         float acc[3][6];
-        // adds the run's sums for triangle `cur` to the table
+        // adds the run's sums for triangle `cur` to the table: the three
+        // vertices' first probes are issued together
         auto flush = [&]() {
+            uint64_t mine[3], old[3];
+            uint32_t h[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const uint32_t vid = cvid[k];
-                uint32_t h = (vid * 0x9E3779B1u) >> 24;   // BWD_TABLE = 256 rows
-                bool found = false;
-                for (uint32_t probe = 0; probe < BWD_PROBES; ++probe) {
-                    const uint32_t old = atomicCAS(&s_key[h], BWD_EMPTY, vid);
-                    if (old == BWD_EMPTY || old == vid) {
-                        found = true;
-                        break;
-                    }
-                    h = (h + 1) & (BWD_TABLE - 1);
+                mine[k] = ((uint64_t)(uint32_t)fix_exponent(acc[k]) << 32) | st.vid[k];
+                h[k] = (st.vid[k] * 0x9E3779B1u) >> 24;   // BWD_TABLE = 256 rows
+                old[k] = atomicCAS((unsigned long long*)&s_key[h[k]], (unsigned long long)BWD_EMPTY,
+                                   (unsigned long long)mine[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t vid = st.vid[k];
+                bool found = old[k] == BWD_EMPTY || (uint32_t)old[k] == vid;
+                for (uint32_t probe = 1; !found && probe < BWD_PROBES; ++probe) {
+                    h[k] = (h[k] + 1) & (BWD_TABLE - 1);
+                    old[k] = atomicCAS((unsigned long long*)&s_key[h[k]], (unsigned long long)BWD_EMPTY,
+                                       (unsigned long long)mine[k]);
+                    found = old[k] == BWD_EMPTY || (uint32_t)old[k] == vid;
                 }
-                if (found) {
-                    float* dst = s_acc + h * 6;
+                // the row's exponent: the claimer's (this lane's when it claimed the row)
+                const int e = (int)(uint32_t)((old[k] == BWD_EMPTY ? mine[k] : old[k]) >> 32);
+                double* dst = p.vert_acc + (size_t)vid * 6;
+                const float lim = __builtin_ldexpf(1.0f, e);
+                const double sc = __builtin_ldexp(1.0, FIX_BITS - e);
 #pragma unroll
-                    for (int m = 0; m < 6; ++m) atomicAdd(dst + m, acc[k][m]);
-                } else {
-                    double* dst = p.vert_acc + (size_t)vid * 6;
-#pragma unroll
-                    for (int m = 0; m < 6; ++m) unsafeAtomicAdd(dst + m, (double)acc[k][m]);
+                for (int m = 0; m < 6; ++m) {
+                    const float t = acc[k][m];
+                    if (found && fabsf(t) < lim)
+                        atomicAdd((unsigned long long*)&s_acc[h[k] * BWD_ROW + m],
+                                  (unsigned long long)__double2ll_rn((double)t * sc));
+                    else
+                        unsafeAtomicAdd(dst + m, (double)t);
                 }
             }
         };
         for (uint32_t row = ty * BWD_TILE_H + threadIdx.x / RUNS; row < min(fr.H, (ty + 1) * BWD_TILE_H);
              row += SHADE_BLOCK / RUNS) {
-            for (uint32_t j = 0; j < BWD_RUN; ++j) {
-                const uint32_t col = col0 + j;
-                if (col >= fr.W) break;
-                const uint64_t g = fr.z_off + (uint64_t)row * fr.W + col;
-                const int32_t v = p.vis[g];
-                if (v < 0) continue;
-                const Fragment f = shade_fragment(p, fr, g, v);
-                const float o0 = f.I * f.c0, o1 = f.I * f.c1, o2 = f.I * f.c2;
-                const float r0 = o0 - p.target[g * 3 + 0];
-                const float r1 = o1 - p.target[g * 3 + 1];
-                const float r2 = o2 - p.target[g * 3 + 2];
-                loss += (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;   // as the forward pass sums it
-                const float g0 = 2.f * r0, g1 = 2.f * r1, g2 = 2.f * r2;
-                const float dI = g0 * f.c0 + g1 * f.c1 + g2 * f.c2;
-                const float* G = f.G;
-                // G^T L (model-space direction of the light for this view)
-                const float gl0 = G[0] * L0 + G[3] * L1 + G[6] * L2;
-                const float gl1 = G[1] * L0 + G[4] * L1 + G[7] * L2;
-                const float gl2 = G[2] * L0 + G[5] * L1 + G[8] * L2;
-                const int32_t T = v >> 1;
-                if (T != cur) {
-                    if (cur >= 0) flush();
-                    cur = T;
+            const uint64_t g_row = fr.z_off + (uint64_t)row * W;
+            for (uint32_t b = 0; b < BWD_RUN; b += BWD_GROUP * cstep) {
+                int32_t vv[BWD_GROUP];
+#pragma unroll
+                for (uint32_t i = 0; i < BWD_GROUP; ++i) {
+                    const uint32_t col = col0 + b + i * cstep;
+                    vv[i] = col < W ? p.vis[g_row + col] : -1;
+                }
+                float tg[BWD_GROUP][3];
+#pragma unroll
+                for (uint32_t i = 0; i < BWD_GROUP; ++i) {
+                    const float* t = p.target + (g_row + col0 + b + i * cstep) * 3;
+                    tg[i][0] = vv[i] >= 0 ? t[0] : 0.0f;
+                    tg[i][1] = vv[i] >= 0 ? t[1] : 0.0f;
+                    tg[i][2] = vv[i] >= 0 ? t[2] : 0.0f;
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < BWD_GROUP; ++i) {
+                    const int32_t v = vv[i];
+                    if (v < 0) continue;
+                    // a thread's run mostly stays on one triangle: its
+                    // set-up, vertex loads and the normal map are redone
+                    // only when the triangle changes
+                    const int32_t T = v >> 1;
+                    if (T != cur) {
+                        if (cur >= 0) flush();
+                        cur = T;
+                        shade_tri(p, fr, (uint32_t)T, st);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+#pragma unroll
+                            for (int m = 0; m < 6; ++m) acc[k][m] = 0.0f;
+                    }
+                    float px, py;
+                    entry_pixel(row, col0 + b + i * cstep, (uint32_t)v & 1u, W, px, py);
+                    const Fragment f = shade_at(p, st, px, py);
+                    const float o0 = f.I * f.c0, o1 = f.I * f.c1, o2 = f.I * f.c2;
+                    const float r0 = o0 - tg[i][0];
+                    const float r1 = o1 - tg[i][1];
+                    const float r2 = o2 - tg[i][2];
+                    loss += (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;   // as the forward pass sums it
+                    const float g0 = 2.f * r0, g1 = 2.f * r1, g2 = 2.f * r2;
+                    const float dI = g0 * f.c0 + g1 * f.c1 + g2 * f.c2;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        cvid[k] = f.vid[k];
-#pragma unroll
-                        for (int m = 0; m < 6; ++m) acc[k][m] = 0.0f;
+                        const float lk = f.lam[k];
+                        acc[k][0] += lk * f.I * g0;
+                        acc[k][1] += lk * f.I * g1;
+                        acc[k][2] += lk * f.I * g2;
+                        acc[k][3] += dI * lk * st.gl[0];
+                        acc[k][4] += dI * lk * st.gl[1];
+                        acc[k][5] += dI * lk * st.gl[2];
                     }
+                    dl0 += (double)(dI * f.n0);
+                    dl1 += (double)(dI * f.n1);
+                    dl2 += (double)(dI * f.n2);
                 }
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float lk = f.lam[k];
-                    acc[k][0] += lk * f.I * g0;
-                    acc[k][1] += lk * f.I * g1;
-                    acc[k][2] += lk * f.I * g2;
-                    acc[k][3] += dI * lk * gl0;
-                    acc[k][4] += dI * lk * gl1;
-                    acc[k][5] += dI * lk * gl2;
-                }
-                dl0 += (double)(dI * f.n0);
-                dl1 += (double)(dI * f.n1);
-                dl2 += (double)(dI * f.n2);
             }
             if (cur >= 0) flush();
             cur = -1;
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < BWD_TABLE; i += SHADE_BLOCK) {
-            const uint32_t vid = s_key[i];
-            if (vid == BWD_EMPTY) continue;
-            double* dst = p.vert_acc + (size_t)vid * 6;
-            for (int m = 0; m < 6; ++m) unsafeAtomicAdd(dst + m, (double)s_acc[i * 6 + m]);
+        {
+            const uint64_t key = s_key[threadIdx.x];
+            if (key != BWD_EMPTY) {
+                const uint32_t vid = (uint32_t)key;
+                const double sc = __builtin_ldexp(1.0, (int)(uint32_t)(key >> 32) - FIX_BITS);
+                double* dst = p.vert_acc + (size_t)vid * 6;
+#pragma unroll
+                for (int m = 0; m < 6; ++m)
+                    unsafeAtomicAdd(dst + m, (double)(int64_t)s_acc[threadIdx.x * BWD_ROW + m] * sc);
+            }
         }
         __syncthreads();   // the table is cleared for the next tile
     }
