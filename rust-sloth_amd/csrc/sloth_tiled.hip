@@ -43,7 +43,6 @@ __global__ void k_tile_clear(uint32_t* __restrict__ cursor, uint64_t n_tiles, in
 // TILED_MAX_TRIS triangles.
 constexpr uint32_t CAREFUL_FLAG = 0x80000000u;
 constexpr uint32_t LOOSE_FLAG = 0x40000000u;
-constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
 
 
 // Tile rectangles (inclusive, in tile units) that a candidate box reaches:
