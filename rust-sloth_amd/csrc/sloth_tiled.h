@@ -7,7 +7,9 @@
 // take the raster's odd instance (described there).
 //
 //   k_tile_clear   tile cursors <- 0, newline rows <- -1, overflow <- 0,
-//                  careful-list length <- 0
+//                  careful-list length <- 0; after sizing and before
+//                  renders of odd-width frames only: k_tile_raster leaves
+//                  what it read in this state for the next render
 //   k_bin          1, 2 or 4 triangles per thread: setup, classification,
 //                  candidate box, newline rows (LDS max per block, one
 //                  atomicMax per row), and the triangle index appended to
@@ -134,7 +136,7 @@ struct TiledPlan {
     int launch_careful(const Frame* d_frames, const Draw* d_draws, bool count_only, uint32_t group, hipStream_t s);
     int launch_raster(const std::vector<Frame>& frames, const Frame* d_frames, const Draw* d_draws,
                       const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells, float* z, int32_t* win,
-                      bool win_raw, uint32_t n_ramps, hipStream_t s);
+                      bool win_raw, uint32_t n_ramps, bool reset_careful, hipStream_t s);
 };
 
 }  // namespace sloth

@@ -1175,27 +1175,44 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
 template <bool BIG, bool ONE, bool ODD>
 __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_eu(BIG && !ONE ? 6 : 7, 8))) k_tile_raster(
     const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
-    int tws, int ths, const uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ pool, const int32_t* __restrict__ nl, const uint4* __restrict__ careful_box,
+    int tws, int ths, uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ pool, int32_t* __restrict__ nl, const uint4* __restrict__ careful_box,
     uint32_t* __restrict__ cells, float* __restrict__ zbuf, int32_t* __restrict__ win, uint32_t win_raw,
-    uint32_t n_ramps, uint4* __restrict__ edge_out)
+    uint32_t n_ramps, uint4* __restrict__ edge_out, uint32_t* __restrict__ careful_reset)
 {
     extern __shared__ uint64_t smem[];
     // the job's first LDS_RAMPS shaders, so that a glyph lookup reads LDS
     // (read only after the tile's first barrier)
     __shared__ Ramp s_ramp[LDS_RAMPS];
 
+    // the careful list was read by k_careful, which ran before this launch
+    if (careful_reset && (blockIdx.x | blockIdx.y | blockIdx.z | threadIdx.x) == 0) careful_reset[0] = 0;
     const Frame& fr = frames[f_lo + blockIdx.z];
     const uint32_t tx = blockIdx.x, ty = blockIdx.y;
     if (tx >= fr.tiles_x || ty >= fr.tiles_y || ((fr.W & 1u) != 0) != ODD) return;
     const uint64_t gt = fr.tile_begin + (uint64_t)ty * fr.tiles_x + tx;
     const uint64_t o = off[gt];
-    const uint32_t n = min(cursor[gt], (uint32_t)(off[gt + 1] - o));
+    const uint32_t c = cursor[gt];
+    const uint32_t n = min(c, (uint32_t)(off[gt + 1] - o));
     if (n > 0)
         for (uint32_t i = threadIdx.x; i < min(n_ramps, LDS_RAMPS) * (uint32_t)sizeof(Ramp) / 4; i += blockDim.x)
             ((uint32_t*)s_ramp)[i] = ((const uint32_t*)ramps)[i];
     raster_tile<BIG, ONE, ODD>(draws, ramps, s_ramp, fr, tx, ty, gt, o, n, tws, ths, pool, nl, careful_box, cells, zbuf,
                                win, win_raw, edge_out, smem);
+    // The tile leaves its binning state as k_tile_clear would for the next
+    // render: its cursor at 0 and, for even widths (k_tile_edges reads odd
+    // frames' rows after this), the newline rows of its tile row back at -1
+    // (the first tile of the row is the one that read them).  Every wave
+    // has read the cursor and the rows once the block has met here.
+    const bool reset_nl = !ODD && fr.image && tx == 0;
+    if (c != 0 || reset_nl) {
+        __syncthreads();
+        if (c != 0 && threadIdx.x == 0) cursor[gt] = 0;
+        if (reset_nl) {
+            const uint32_t r0 = ty << ths, r1 = min(fr.H, r0 + (1u << ths));
+            for (uint32_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) nl[fr.nl_off + r] = -1;
+        }
+    }
 }
 
 // Frames of odd width: the first-column cells of the tiles.  Cell (r, c)
@@ -1403,6 +1420,8 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     }
     if ((rc = d_pool.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, off[n_tiles])))) return rc;
     HIP_TRY(hipMemcpyAsync(d_off.p, off.data(), sizeof(uint64_t) * (n_tiles + 1), hipMemcpyHostToDevice, s));
+    // renders start from cleared cursors, rows and careful list (each leaves them so)
+    if ((rc = launch_clear(frames, 0, F, s))) return rc;
     HIP_TRY(hipStreamSynchronize(s));
     sized = true;
     ++generation;
@@ -1411,8 +1430,11 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
 
 int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_frames, const Draw* d_draws,
                              const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells, float* z, int32_t* win,
-                             bool win_raw, uint32_t n_ramps, hipStream_t s)
+                             bool win_raw, uint32_t n_ramps, bool reset_careful, hipStream_t s)
 {
+    // the first raster launch resets the careful list's length (k_careful
+    // has read it) when asked
+    uint32_t* careful_reset = reset_careful ? d_careful_cnt.as<uint32_t>() : nullptr;
     // grid (tile x, tile y, frame) over the largest tile grid of the range,
     // at most 65535 frames per launch
     for (uint32_t f0 = f_lo; f0 < f_hi; f0 += 65535u) {
@@ -1436,8 +1458,10 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
             hipLaunchKernelGGL(kern, dim3(mx, my, f1 - f0), dim3(RASTER_BLOCK), raster_lds(false), s, d_draws,
                                d_ramps, d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
                                d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
-                               d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps, (uint4*)nullptr);
+                               d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps, (uint4*)nullptr,
+                               careful_reset);
             HIP_TRY(hipGetLastError());
+            careful_reset = nullptr;
         }
         if (od) {
             const auto kern = one ? k_tile_raster<false, true, true> : k_tile_raster<false, false, true>;
@@ -1445,8 +1469,9 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
                                d_ramps, d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
                                d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
                                d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps,
-                               d_edges.as<uint4>());
+                               d_edges.as<uint4>(), careful_reset);
             HIP_TRY(hipGetLastError());
+            careful_reset = nullptr;
             uint32_t mh = 0;
             for (uint32_t f = f0; f < f1; ++f) mh = std::max(mh, frames[f].H);
             hipLaunchKernelGGL(k_tile_edges, dim3(blocks(mh, 256), mx, f1 - f0), dim3(256), 0, s, d_frames, f0,
@@ -1455,6 +1480,8 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
             HIP_TRY(hipGetLastError());
         }
     }
+    // no raster block ran (frames without tiles): reset the length here
+    if (careful_reset) return launch_clear(frames, f_lo, f_hi, s);
     return SLOTH_OK;
 }
 
@@ -1473,14 +1500,22 @@ int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>&
     uint32_t groups = 1;
     if (s2 && join && !ev) groups = std::min<uint32_t>({MAX_GROUPS, nf / PIPE_MIN_FRAMES, (uint32_t)join->size() - 1});
     if (groups < 2) {
+        // Every render leaves the tile cursors, the even frames' newline
+        // rows and the careful list's length as k_tile_clear sets them
+        // (k_tile_raster resets what it has read; size() ends with a
+        // clear), so only ranges with odd widths, whose newline rows
+        // k_tile_edges reads after the raster, are cleared first.
+        bool odd = false;
+        for (uint32_t f = f_lo; f < f_hi; ++f) odd = odd || (frames[f].W & 1u);
         if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
-        if ((rc = launch_clear(frames, f_lo, f_hi, s))) return rc;
+        if (odd && (rc = launch_clear(frames, f_lo, f_hi, s))) return rc;
         if (ev) HIP_TRY(hipEventRecord((*ev)[1], s));
         if ((rc = launch_bin(frames, draws, d_frames, d_draws, f_lo, f_hi, false, s))) return rc;
         if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
         if ((rc = launch_careful(d_frames, d_draws, false, 0, s))) return rc;
         if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
-        if ((rc = launch_raster(frames, d_frames, d_draws, d_ramps, f_lo, f_hi, cells, z, win, win_raw, n_ramps, s)))
+        if ((rc = launch_raster(frames, d_frames, d_draws, d_ramps, f_lo, f_hi, cells, z, win, win_raw, n_ramps, true,
+                                s)))
             return rc;
         if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));
         return SLOTH_OK;
@@ -1496,7 +1531,9 @@ int TiledPlan::launch(const std::vector<Frame>& frames, const std::vector<Draw>&
         if ((rc = launch_careful(d_frames, d_draws, false, g, s))) return rc;
         HIP_TRY(hipEventRecord((*join)[g], s));
         HIP_TRY(hipStreamWaitEvent(s2, (*join)[g], 0));
-        if ((rc = launch_raster(frames, d_frames, d_draws, d_ramps, a, b, cells, z, win, win_raw, n_ramps, s2)))
+        // the last group's raster runs after every group's careful pass
+        if ((rc = launch_raster(frames, d_frames, d_draws, d_ramps, a, b, cells, z, win, win_raw, n_ramps,
+                                g + 1 == groups, s2)))
             return rc;
     }
     HIP_TRY(hipEventRecord((*join)[groups], s2));
