@@ -65,7 +65,7 @@ struct Tri {
     float x1, y1, z1, x2, y2, z2, x3, y3, z3;
     float A0, B0, A1, B1, A2, B2;   // edge deltas of orient(v2,v3,.), orient(v3,v1,.), orient(v1,v2,.)
     float a;                        // 1 / orient(v1,v2,v3)      rasterizer.rs:67
-    float minx, maxx;
+    float minx, maxx, miny, maxy;   // the vertices' extent
     int x0, x1d, y0, y1d;           // pixel domain [x0,x1d) x [y0,y1d)  rasterizer.rs:58-66
 };
 
@@ -116,12 +116,12 @@ __device__ __forceinline__ void setup_tri_from(const float* M, const P& p, uint3
     t.z3 = xform_row(M, 2, p[6], p[7], p[8]);
     t.minx = fminf(t.x1, fminf(t.x2, t.x3));
     t.maxx = fmaxf(t.x1, fmaxf(t.x2, t.x3));
-    float miny = fminf(t.y1, fminf(t.y2, t.y3));
-    float maxy = fmaxf(t.y1, fmaxf(t.y2, t.y3));
+    t.miny = fminf(t.y1, fminf(t.y2, t.y3));
+    t.maxy = fmaxf(t.y1, fmaxf(t.y2, t.y3));
     t.x0 = sat_index(ceilf(fmaxf(t.minx, 1.0f)));
-    t.y0 = sat_index(ceilf(fmaxf(miny, 1.0f)));
+    t.y0 = sat_index(ceilf(fmaxf(t.miny, 1.0f)));
     t.x1d = sat_index(ceilf(fminf(t.maxx * 2.0f, (float)(W - 1))));
-    t.y1d = sat_index(ceilf(fminf(maxy, (float)(H - 1))));
+    t.y1d = sat_index(ceilf(fminf(t.maxy, (float)(H - 1))));
     t.A0 = t.x3 - t.x2;
     t.B0 = t.y3 - t.y2;
     t.A1 = t.x1 - t.x3;
@@ -283,12 +283,27 @@ __device__ __forceinline__ void key_pixel(uint32_t id, uint32_t sub, uint32_t W,
 // Classification of a triangle's candidate pixels (see setup_class).
 enum : uint32_t { CLS_EMPTY = 0, CLS_BOX = 1, CLS_SPANS = 2, CLS_FULL = 3, CLS_TIGHT = 4 };
 
+// An upper bound of sum_k |A_k| |py - ay_k| + |B_k| |px - ax_k| (edge k
+// anchored at vertex (ax_k, ay_k)) over the pixels of [X0, X1] x [Y0, Y1],
+// X0 <= X1 and Y0 <= Y1: for a vertex x in [minx, maxx], |px - x| <=
+// DX = max(|X1 - minx|, |X0 - maxx|) (|X0 - x| <= X1 - minx when x <= X0,
+// <= maxx - X0 otherwise; likewise for X1), so the sum is at most
+// (sum |A_k|) DY + (sum |B_k|) DX.  Its few roundings (relative error 2^-24
+// each) are within the margins of the bounds that use it.
+__device__ __forceinline__ float edge_terms_bound(const Tri& t, float X0, float X1, float Y0, float Y1)
+{
+    const float DX = fmaxf(fabsf(X1 - t.minx), fabsf(X0 - t.maxx));
+    const float DY = fmaxf(fabsf(Y1 - t.miny), fabsf(Y0 - t.maxy));
+    return (fabsf(t.A0) + fabsf(t.A1) + fabsf(t.A2)) * DY + (fabsf(t.B0) + fabsf(t.B1) + fabsf(t.B2)) * DX;
+}
+
 // Conservative proof that the reference's FP32 edge test cannot pass outside
 // the box [floor(minx), ceil(maxx)] (CLS_BOX), outside the tight box
 // [ceil(minx - 1/32), floor(maxx + 1/32)] (CLS_TIGHT), or anywhere at all (back
 // face, CLS_EMPTY).  Uses |w_fp - w_exact| <= 4.0001 u (|A| |py-ay| + |B|
-// |px-ax|) per edge over the whole pixel domain, and that the exact edge
-// functions sum to the exact orient(v1,v2,v3) = o.
+// |px-ax|) per edge over the whole pixel domain (summed over the edges by
+// edge_terms_bound), and that the exact edge functions sum to the exact
+// orient(v1,v2,v3) = o.
 //
 // Box argument: a pixel d to the left of minx (or right of maxx) has some
 // exact barycentric lambda_i <= -d / (2 (maxx - minx)) (at most two of the
@@ -318,13 +333,9 @@ __device__ __forceinline__ uint32_t setup_class(const Tri& t)
     const float my = __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(t.y1), fabsf(t.y2)), fabsf(t.y3));
     const bool finite = __builtin_elementwise_maximum(mx, my) < LIM;
     const float K = 8.0f * 5.9604645e-08f;
-    float X0 = (float)t.x0, X1 = (float)(t.x1d - 1), Y0 = (float)t.y0, Y1 = (float)(t.y1d - 1);
-    float dx0 = fmaxf(fabsf(X0 - t.x2), fabsf(X1 - t.x2)), dy0 = fmaxf(fabsf(Y0 - t.y2), fabsf(Y1 - t.y2));
-    float dx1 = fmaxf(fabsf(X0 - t.x3), fabsf(X1 - t.x3)), dy1 = fmaxf(fabsf(Y0 - t.y3), fabsf(Y1 - t.y3));
-    float dx2 = fmaxf(fabsf(X0 - t.x1), fabsf(X1 - t.x1)), dy2 = fmaxf(fabsf(Y0 - t.y1), fabsf(Y1 - t.y1));
-    float S = fabsf(t.A0) * dy0 + fabsf(t.B0) * dx0 + fabsf(t.A1) * dy1 + fabsf(t.B1) * dx1 +
-              fabsf(t.A2) * dy2 + fabsf(t.B2) * dx2;
-    float bound = K * S;
+    // a non-empty domain has X0 <= X1 and Y0 <= Y1 (an empty one is CLS_EMPTY)
+    const float X0 = (float)t.x0, X1 = (float)(t.x1d - 1), Y0 = (float)t.y0, Y1 = (float)(t.y1d - 1);
+    const float bound = K * edge_terms_bound(t, X0, X1, Y0, Y1);
     float ey = t.y3 - t.y1, ex = t.x3 - t.x1;
     float o = t.A2 * ey - t.B2 * ex;
     float eo = K * (fabsf(t.A2) * fabsf(ey) + fabsf(t.B2) * fabsf(ex));

@@ -881,8 +881,9 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         // sum to the exact orient, the two additions add at most 2.0001u of
         // their inputs' magnitudes, and the FP32 orient o errs from the exact
         // one by at most 4.0002u (|A2| |y3 - y1| + |B2| |x3 - x1|).  So every
-        // S lies within E = 10u (Sb + Q) of o, with Sb the edge terms' sum
-        // over the box and Q o's terms; 10u also covers the rounding of E
+        // S lies within E = 10u (Sb + Q) of o, with Sb a bound of the edge
+        // terms' sum over the box (edge_terms_bound) and Q o's terms; 10u
+        // also covers the rounding of E
         // and of o -+ E (Sb >= |o|).  fl(na * .) and the step index are
         // monotone, so equal indices at the two ends mean one glyph for the
         // whole box.  Shaders that are not non-decreasing, and non-finite
@@ -898,11 +899,8 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         // it is at most thr[i] (thresholds are non-decreasing).
         auto box_glyph = [&](const Ramp& rp, const Tri& t, float na, int bx0, int bx1, int by0, int by1) -> uint32_t {
             if (!rp.sorted) return ~0u;
-            const float X0 = (float)bx0, X1 = (float)bx1, Y0 = (float)by0, Y1 = (float)by1;
-            const float Sb =
-                fabsf(t.A0) * fmaxf(fabsf(Y0 - t.y2), fabsf(Y1 - t.y2)) + fabsf(t.B0) * fmaxf(fabsf(X0 - t.x2), fabsf(X1 - t.x2)) +
-                fabsf(t.A1) * fmaxf(fabsf(Y0 - t.y3), fabsf(Y1 - t.y3)) + fabsf(t.B1) * fmaxf(fabsf(X0 - t.x3), fabsf(X1 - t.x3)) +
-                fabsf(t.A2) * fmaxf(fabsf(Y0 - t.y1), fabsf(Y1 - t.y1)) + fabsf(t.B2) * fmaxf(fabsf(X0 - t.x1), fabsf(X1 - t.x1));
+            // (an empty box has no pixels, so its glyph is never used)
+            const float Sb = edge_terms_bound(t, (float)bx0, (float)bx1, (float)by0, (float)by1);
             const float ey = t.y3 - t.y1, ex = t.x3 - t.x1;
             const float o = t.A2 * ey - t.B2 * ex;
             const float E = 5.9604645e-07f * (Sb + fabsf(t.A2) * fabsf(ey) + fabsf(t.B2) * fabsf(ex));

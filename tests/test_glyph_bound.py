@@ -77,7 +77,8 @@ def _setup(t, W, H):
     bx0 = np.maximum(x0, np.ceil(minx - f32(0.03125)).astype(np.int64))
     bx1 = np.minimum(x1d - 1, np.floor(maxx + f32(0.03125)).astype(np.int64))
     return dict(x1=x1, y1=y1, x2=x2, y2=y2, x3=x3, y3=y3, A0=A0, B0=B0, A1=A1, B1=B1, A2=A2, B2=B2, o=o, na=na,
-                nz=nz, n2=n2, a=a, bx0=bx0, bx1=bx1, by0=y0, by1=y1d - 1)
+                nz=nz, n2=n2, a=a, bx0=bx0, bx1=bx1, by0=y0, by1=y1d - 1, minx=minx, maxx=maxx, miny=miny,
+                maxy=maxy)
 
 
 def _na_hardware(s, eps):
@@ -95,12 +96,12 @@ def _box_glyph_index(s, na_h):
     X0, X1 = s["bx0"].astype(np.float32), s["bx1"].astype(np.float32)
     Y0, Y1 = s["by0"].astype(np.float32), s["by1"].astype(np.float32)
 
-    def term(A, B, ax, ay):
-        return np.abs(A) * np.maximum(np.abs(Y0 - ay), np.abs(Y1 - ay)) + \
-            np.abs(B) * np.maximum(np.abs(X0 - ax), np.abs(X1 - ax))
-
-    Sb = (((((term(s["A0"], s["B0"], s["x2"], s["y2"])))) + term(s["A1"], s["B1"], s["x3"], s["y3"]))
-          + term(s["A2"], s["B2"], s["x1"], s["y1"]))
+    # edge_terms_bound (sloth_device.h)
+    DX = np.maximum(np.abs(X1 - s["minx"]), np.abs(X0 - s["maxx"]))
+    DY = np.maximum(np.abs(Y1 - s["miny"]), np.abs(Y0 - s["maxy"]))
+    SA = (np.abs(s["A0"]) + np.abs(s["A1"])) + np.abs(s["A2"])
+    SB = (np.abs(s["B0"]) + np.abs(s["B1"])) + np.abs(s["B2"])
+    Sb = SA * DY + SB * DX
     ey, ex = s["y3"] - s["y1"], s["x3"] - s["x1"]
     o = s["A2"] * ey - s["B2"] * ex
     E = f32(5.9604645e-07) * ((Sb + np.abs(s["A2"]) * np.abs(ey)) + np.abs(s["B2"]) * np.abs(ex))
@@ -155,3 +156,35 @@ def test_constant_glyph_decision_holds_per_pixel(eps):
     far = nonempty & ~near
     assert (claimed & far).sum() >= 0.99 * far.sum(), ((claimed & far).sum(), far.sum())
     assert (claimed & near & nonempty).sum() >= 1000
+
+
+def test_box_terms_bound_covers_the_exact_sum():
+    """The decision's Sb (edge_terms_bound over the entry's box) is at least
+    the exact maximum over the box of sum |A| |py - ay| + |B| |px - ax|,
+    which a sum of convex functions takes at a corner of the box; a smaller
+    Sb would narrow the shade interval below what the rounding allows."""
+    rng = np.random.default_rng(99)
+    t, W, H, _ = _triangles(20000, rng)
+    s = _setup(t, W, H)
+    ok = (s["bx0"] <= s["bx1"]) & (s["by0"] <= s["by1"])
+    X0, X1 = s["bx0"].astype(np.float32), s["bx1"].astype(np.float32)
+    Y0, Y1 = s["by0"].astype(np.float32), s["by1"].astype(np.float32)
+    DX = np.maximum(np.abs(X1 - s["minx"]), np.abs(X0 - s["maxx"]))
+    DY = np.maximum(np.abs(Y1 - s["miny"]), np.abs(Y0 - s["maxy"]))
+    Sb = ((np.abs(s["A0"]) + np.abs(s["A1"])) + np.abs(s["A2"])) * DY + \
+        ((np.abs(s["B0"]) + np.abs(s["B1"])) + np.abs(s["B2"])) * DX
+    f = lambda k: s[k].astype(np.float64)
+    edges = ((f("x2"), f("y2"), f("x3") - f("x2"), f("y3") - f("y2")),
+             (f("x3"), f("y3"), f("x1") - f("x3"), f("y1") - f("y3")),
+             (f("x1"), f("y1"), f("x2") - f("x1"), f("y2") - f("y1")))
+    corner = np.zeros(len(t))
+    for cx in (s["bx0"].astype(np.float64), s["bx1"].astype(np.float64)):
+        for cy in (s["by0"].astype(np.float64), s["by1"].astype(np.float64)):
+            tot = sum(np.abs(A) * np.abs(cy - ay) + np.abs(B) * np.abs(cx - ax) for ax, ay, A, B in edges)
+            corner = np.maximum(corner, tot)
+    assert ok.sum() > 10000
+    ratio = Sb[ok].astype(np.float64) / np.maximum(corner[ok], 1e-300)
+    assert (ratio >= 1 - 1e-6).all(), ratio.min()
+    # and within 4x of the exact sum throughout (boxes of a pixel or two,
+    # where the vertices' spread sets DX and DY, are the loosest)
+    assert ratio.max() < 4.0, ratio.max()
