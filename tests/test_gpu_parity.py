@@ -314,6 +314,59 @@ def _slivers(rng, n, spread=1.0):
     return xyz, rng.integers(0, 256, size=(n, 3), dtype=np.uint8)
 
 
+def test_binning_state_across_launches(sloth, raster_path):
+    """Renders of even-width frames launch no clear kernel: each raster
+    block leaves its tile cursor, its newline rows and the careful list's
+    length as the clear sets them.  Graph replays, per-frame launches, a
+    re-planned tile size, a one-frame re-render for winners and new
+    transforms follow one another here (image frames with newline rows,
+    slivers that take the careful pass), every frame against the oracle
+    after each."""
+    from sloth_amd import meshgen
+
+    if raster_path == "global":
+        pytest.skip("the binning state belongs to the tiled path")
+    rng = np.random.default_rng(77)
+    m = meshgen.displaced_icosphere(3, seed=5)
+    sm = sloth.SimpleMesh.from_mesh(m)
+    sxyz, srgb = _slivers(rng, 60)
+    sliv = sloth.SimpleMesh(sxyz, srgb)
+    sizes = [(96, 40, True), (128, 33, True), (64, 64, False), (200, 20, True)]
+    frames, draws, plan = [], [], []
+    for f, (W, H, image) in enumerate(sizes):
+        U = meshgen.screen_transform(W, H, [m.bbox_max(), np.array([1.0, 1.0, 1.0], np.float32)])
+        frames.append(sloth.FrameSpec(W, H, U, image))
+        for mesh, xyz, rgb in ((sm, m.xyz, m.rgb), (sliv, sxyz, srgb)):
+            R = meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3))
+            draws.append(sloth.DrawSpec(f, mesh, R))
+            plan.append((f, xyz, rgb, R))
+    b = sloth.Batch(frames, draws)
+
+    def check(step, winners_of=None):
+        say(f"case binning state: {step}")
+        for f, (W, H, image) in enumerate(sizes):
+            U = frames[f].utransform
+            ref = oracle_render(W, H, U, [(xyz, rgb, R, None) for (ff, xyz, rgb, R) in plan if ff == f], image=image)
+            cells, z, win = b.read_frame(f, want_winners=(f == winners_of))
+            assert_frame_equal(cells, z, win if f == winners_of else None, ref, W)
+
+    for step in ("graph", "per-frame", "graph again"):
+        b.run_per_frame() if step == "per-frame" else b.run()
+        b.sync()
+        check(step, winners_of=1 if step == "per-frame" else None)
+    b.set_tile(64, 16)
+    b.run()
+    b.sync()
+    check("re-planned 64x16 tiles", winners_of=2)
+    new = [meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3)) for _ in plan]
+    plan = [(f, xyz, rgb, R) for (f, xyz, rgb, _), R in zip(plan, new)]
+    b.set_transforms(np.stack([R.reshape(16) for R in new]))
+    for step in ("new transforms", "new transforms, replayed"):
+        b.run()
+        b.sync()
+        check(step)
+
+
 @pytest.mark.parametrize("n_frames,pipeline", [(40, True), (136, True), (136, False)])
 def test_batch_pipelined_groups(sloth, n_frames, pipeline, monkeypatch):
     """With SLOTH_PIPELINE=1, graph launches of 32+ frames bin one frame group
