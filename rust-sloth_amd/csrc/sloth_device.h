@@ -101,6 +101,8 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* global_ptr
     return (const __attribute__((address_space(1))) T*)p;
 }
 
+__device__ __forceinline__ void tri_finish(uint32_t W, uint32_t H, Tri& t);
+
 // Transformed triangle from its 9 model-space coordinates p (v1, v2, v3).
 template <class P>
 __device__ __forceinline__ void setup_tri_from(const float* M, const P& p, uint32_t W, uint32_t H, Tri& t)
@@ -114,6 +116,13 @@ __device__ __forceinline__ void setup_tri_from(const float* M, const P& p, uint3
     t.x3 = xform_row(M, 0, p[6], p[7], p[8]);
     t.y3 = xform_row(M, 1, p[6], p[7], p[8]);
     t.z3 = xform_row(M, 2, p[6], p[7], p[8]);
+    tri_finish(W, H, t);
+}
+
+// The rest of Triangle set-up from the screen vertices t.x*, t.y*: extent,
+// pixel domain, edge deltas and a (rasterizer.rs:58-67).
+__device__ __forceinline__ void tri_finish(uint32_t W, uint32_t H, Tri& t)
+{
     t.minx = fminf(t.x1, fminf(t.x2, t.x3));
     t.maxx = fmaxf(t.x1, fmaxf(t.x2, t.x3));
     t.miny = fminf(t.y1, fminf(t.y2, t.y3));

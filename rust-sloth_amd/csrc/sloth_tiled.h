@@ -68,8 +68,9 @@ struct TiledPlan {
     uint64_t generation = 0;         // bumped whenever size() runs (device pointers may change)
     std::vector<uint64_t> off;       // host copy of the per-tile pool offsets (n_tiles + 1)
     DevBuf d_cursor, d_off, d_pool, d_nl, d_overflow;
-    // triangles whose candidate box k_careful computes: (draw, local index)
-    // in, (triangle, box) out; capacity from the counting run plus headroom
+    // triangles whose candidate box k_careful computes: (frame, triangle,
+    // screen vertices) in, 32 bytes each, (triangle, box) out; capacity from
+    // the counting run plus headroom
     DevBuf d_careful_cnt, d_careful_in, d_careful_box;
     uint32_t careful_cap = 0;
     // set by size(): the lists hold enough big boxes (more than 128 pixels,

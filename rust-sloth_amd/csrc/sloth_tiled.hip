@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
                                                    uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
                                                    uint32_t* __restrict__ pool, int32_t* __restrict__ nl,
                                                    uint32_t* __restrict__ overflow, uint32_t* __restrict__ careful_cnt,
-                                                   uint2* __restrict__ careful_in, uint32_t careful_cap)
+                                                   uint4* __restrict__ careful_in, uint32_t careful_cap)
 {
     __shared__ uint32_t w_cnt[BIN_WIN * BIN_WIN];
     __shared__ uint64_t w_base[BIN_WIN * BIN_WIN];   // pool index of the tile's reserved run
@@ -307,7 +307,13 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         geo[k] = g;
         if (careful) {
             const uint32_t idx = atomicAdd(careful_cnt, 1u);
-            if (idx < careful_cap) careful_in[idx] = make_uint2(d, local);
+            if (idx < careful_cap) {
+                // the screen vertices come along, so that k_careful starts
+                // from them instead of the draw record and the mesh
+                careful_in[2 * idx] = make_uint4(frame, dr.tri_begin + local, __float_as_uint(t.x1), __float_as_uint(t.y1));
+                careful_in[2 * idx + 1] = make_uint4(__float_as_uint(t.x2), __float_as_uint(t.y2),
+                                                     __float_as_uint(t.x3), __float_as_uint(t.y3));
+            }
             else if (!COUNT) overflow[frame] = 1u;
         }
         const bool has_rows = fr.image && valid && t.y0 < t.y1d;
@@ -467,19 +473,25 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 // wave's min/max; CLS_FULL: the whole domain.  The lanes then append the
 // triangle to every tile of the box, and lane 0 stores the box for the raster.
 template <bool COUNT>
-__device__ __forceinline__ void careful_one(const Draw* __restrict__ draws, const Frame* __restrict__ frames, int tws,
+__device__ __forceinline__ void careful_one(const Frame* __restrict__ frames, int tws,
                                             int ths, uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
                                             uint32_t* __restrict__ pool, uint32_t* __restrict__ overflow,
-                                            const uint2* __restrict__ careful_in, uint4* __restrict__ careful_box,
+                                            const uint4* __restrict__ careful_in, uint4* __restrict__ careful_box,
                                             uint32_t w, int lane)
 {
-    const uint2 in = careful_in[w];
-    const Draw& dr = draws[in.x];
-    const uint32_t frame = dr.frame;
+    // (frame, frame triangle index, screen vertices as k_bin transformed them)
+    const uint4 in = careful_in[2 * w], in2 = careful_in[2 * w + 1];
+    const uint32_t frame = in.x;
     const Frame& fr = frames[frame];
     Tri t;
-    setup_draw_tri(dr, in.y, fr.W, fr.H, t);
-    const uint32_t T = dr.tri_begin + in.y;
+    t.x1 = __uint_as_float(in.z);
+    t.y1 = __uint_as_float(in.w);
+    t.x2 = __uint_as_float(in2.x);
+    t.y2 = __uint_as_float(in2.y);
+    t.x3 = __uint_as_float(in2.z);
+    t.y3 = __uint_as_float(in2.w);
+    tri_finish(fr.W, fr.H, t);
+    const uint32_t T = in.y;
     const uint32_t cls = setup_class(t);
     int bx0 = t.x0, bx1 = t.x1d - 1, by0 = t.y0, by1 = t.y1d - 1;
     if (cls == CLS_SPANS) {
@@ -531,11 +543,10 @@ __device__ __forceinline__ void careful_one(const Draw* __restrict__ draws, cons
 // Group g takes [careful_cnt[1 + g], length) and records the length as group
 // g + 1's start.
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws, const Frame* __restrict__ frames,
-                                                 int tws, int ths, uint32_t* __restrict__ cursor,
+__global__ void __launch_bounds__(256) k_careful(const Frame* __restrict__ frames, int tws, int ths, uint32_t* __restrict__ cursor,
                                                  const uint64_t* __restrict__ off, uint32_t* __restrict__ pool,
                                                  uint32_t* __restrict__ overflow, uint32_t* __restrict__ careful_cnt,
-                                                 uint32_t group, const uint2* __restrict__ careful_in,
+                                                 uint32_t group, const uint4* __restrict__ careful_in,
                                                  uint4* __restrict__ careful_box, uint32_t careful_cap)
 {
     const int lane = (int)__lane_id();
@@ -544,7 +555,7 @@ __global__ void __launch_bounds__(256) k_careful(const Draw* __restrict__ draws,
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
     if (blockIdx.x == 0 && threadIdx.x == 0) careful_cnt[2 + group] = n;
     for (uint32_t w = first + ((blockIdx.x * blockDim.x + threadIdx.x) >> 6); w < n; w += n_waves)
-        careful_one<COUNT>(draws, frames, tws, ths, cursor, off, pool, overflow, careful_in, careful_box, w, lane);
+        careful_one<COUNT>(frames, tws, ths, cursor, off, pool, overflow, careful_in, careful_box, w, lane);
 }
 
 
@@ -1346,7 +1357,7 @@ int TiledPlan::launch_bin(const std::vector<Frame>& frames, const std::vector<Dr
             hipLaunchKernelGGL(kernel, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
                                d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
                                d_nl.as<int32_t>(), d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(),
-                               d_careful_in.as<uint2>(), careful_cap);
+                               d_careful_in.as<uint4>(), careful_cap);
         };
         if (bin_tpt == 1) count_only ? go(k_bin<true, 1>) : go(k_bin<false, 1>);
         else if (bin_tpt == 2) count_only ? go(k_bin<true, 2>) : go(k_bin<false, 2>);
@@ -1364,14 +1375,14 @@ int TiledPlan::launch_careful(const Frame* d_frames, const Draw* d_draws, bool c
     // a fixed grid of wavefronts strides over the careful list (one triangle per wavefront at a time)
     dim3 grid(std::min(blocks(careful_cap, 4), 1024u));
     if (count_only)
-        hipLaunchKernelGGL(k_careful<true>, grid, dim3(256), 0, s, d_draws, d_frames, tws, ths,
+        hipLaunchKernelGGL(k_careful<true>, grid, dim3(256), 0, s, d_frames, tws, ths,
                            d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
-                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), group, d_careful_in.as<uint2>(),
+                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), group, d_careful_in.as<uint4>(),
                            d_careful_box.as<uint4>(), careful_cap);
     else
-        hipLaunchKernelGGL(k_careful<false>, grid, dim3(256), 0, s, d_draws, d_frames, tws, ths,
+        hipLaunchKernelGGL(k_careful<false>, grid, dim3(256), 0, s, d_frames, tws, ths,
                            d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
-                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), group, d_careful_in.as<uint2>(),
+                           d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(), group, d_careful_in.as<uint4>(),
                            d_careful_box.as<uint4>(), careful_cap);
     HIP_TRY(hipGetLastError());
     return SLOTH_OK;
@@ -1399,7 +1410,7 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     if (n_careful > careful_cap) {
         // 2. room for the careful list, then count again so that it is filled
         careful_cap = n_careful + n_careful / 4 + 256;
-        if ((rc = d_careful_in.reserve(sizeof(uint2) * careful_cap))) return rc;
+        if ((rc = d_careful_in.reserve(2 * sizeof(uint4) * careful_cap))) return rc;
         if ((rc = d_careful_box.reserve(sizeof(uint4) * careful_cap))) return rc;
         if ((rc = launch_clear(frames, 0, F, s))) return rc;
         if ((rc = launch_bin(frames, draws, d_frames, d_draws, 0, F, true, s))) return rc;
