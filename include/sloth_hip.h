@@ -270,7 +270,9 @@ int sloth_batch_device_frame(sloth_batch *b, uint32_t frame, uint32_t **d_cells,
 
 /* Times each stage of one (non-graph) run with HIP events on the batch's
  * stream: out_ms[i] = milliseconds of stage i, names[i] its kernel name;
- * *n_stages in: capacity, out: count. */
+ * *n_stages in: capacity, out: count.  On the tiled path the first stage
+ * (k_tile_clear) runs only for batches with frames of odd width; otherwise
+ * it times two adjacent events. */
 int sloth_batch_profile(sloth_batch *b, float *out_ms, const char **names, uint32_t *n_stages);
 
 /* Runs `warmup` untimed renders, then `steps` timed renders bracketed by HIP
