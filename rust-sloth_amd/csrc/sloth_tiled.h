@@ -73,6 +73,15 @@ struct TiledPlan {
     // the counting run plus headroom
     DevBuf d_careful_cnt, d_careful_in, d_careful_box;
     uint32_t careful_cap = 0;
+    // Batches whose draws differ in size, or whose frames differ in tile
+    // grid, launch flat grids instead of grids padded to the largest draw
+    // (k_bin) or tile grid (the raster), whose extra blocks would only exit:
+    // k_bin takes one block per (draw, BIN_BLOCK * TPT triangles) from a
+    // table per TPT, the raster one workgroup per tile of the range.
+    bool flat_bin = false, flat_raster = false;
+    DevBuf d_bin_blocks[3];                      // per TPT 1, 2, 4: (draw, first triangle) of each block
+    std::vector<uint64_t> bin_block_begin[3];    // per TPT: each draw's first block (draws + 1)
+    DevBuf d_tile_frame;                         // each tile's frame
     // set by size(): the lists hold enough big boxes (more than 128 pixels,
     // one triangle in 512 or more) for the raster variant whose wavefronts
     // walk those boxes together; SLOTH_BIG_WALK=0/1 overrides

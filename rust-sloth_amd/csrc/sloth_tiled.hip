@@ -215,7 +215,8 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
                                                    uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
                                                    uint32_t* __restrict__ pool, int32_t* __restrict__ nl,
                                                    uint32_t* __restrict__ overflow, uint32_t* __restrict__ careful_cnt,
-                                                   uint4* __restrict__ careful_in, uint32_t careful_cap)
+                                                   uint4* __restrict__ careful_in, uint32_t careful_cap,
+                                                   const uint2* __restrict__ flat_blocks)
 {
     __shared__ uint32_t w_cnt[BIN_WIN * BIN_WIN];
     __shared__ uint64_t w_base[BIN_WIN * BIN_WIN];   // pool index of the tile's reserved run
@@ -223,9 +224,18 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     __shared__ int32_t s_nl[BIN_BLOCK];
     __shared__ int s_tx0, s_ty0, s_lo, s_hi;
 
-    const uint32_t d = d_lo + blockIdx.y;
+    // the block's draw and first triangle: from the grid, or (flat grids of
+    // draws of different sizes) from its table entry
+    uint32_t d, base;
+    if (flat_blocks) {
+        const uint2 e = flat_blocks[blockIdx.x];
+        d = e.x;
+        base = e.y;
+    } else {
+        d = d_lo + blockIdx.y;
+        base = blockIdx.x * (BIN_BLOCK * TPT);
+    }
     const Draw& dr = draws[d];
-    const uint32_t base = blockIdx.x * (BIN_BLOCK * TPT);
     if (base >= dr.n) return;
     const uint32_t frame = dr.frame;
     const Frame& fr = frames[frame];
@@ -1187,7 +1197,8 @@ __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_
     int tws, int ths, uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ pool, int32_t* __restrict__ nl, const uint4* __restrict__ careful_box,
     uint32_t* __restrict__ cells, float* __restrict__ zbuf, int32_t* __restrict__ win, uint32_t win_raw,
-    uint32_t n_ramps, uint4* __restrict__ edge_out, uint32_t* __restrict__ careful_reset)
+    uint32_t n_ramps, uint4* __restrict__ edge_out, uint32_t* __restrict__ careful_reset,
+    const uint32_t* __restrict__ tile_frame, uint32_t t_lo)
 {
     extern __shared__ uint64_t smem[];
     // the job's first LDS_RAMPS shaders, so that a glyph lookup reads LDS
@@ -1196,8 +1207,21 @@ __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_
 
     // the careful list was read by k_careful, which ran before this launch
     if (careful_reset && (blockIdx.x | blockIdx.y | blockIdx.z | threadIdx.x) == 0) careful_reset[0] = 0;
-    const Frame& fr = frames[f_lo + blockIdx.z];
-    const uint32_t tx = blockIdx.x, ty = blockIdx.y;
+    // the block's tile: (x, y, frame) from a 3-D grid, or (frames of
+    // different tile grids) tile t_lo + x of a flat one
+    uint32_t f, tx, ty;
+    if (tile_frame) {
+        const uint32_t g = t_lo + blockIdx.x;
+        f = tile_frame[g];
+        const uint32_t local = g - (uint32_t)frames[f].tile_begin;
+        ty = local / frames[f].tiles_x;
+        tx = local - ty * frames[f].tiles_x;
+    } else {
+        f = f_lo + blockIdx.z;
+        tx = blockIdx.x;
+        ty = blockIdx.y;
+    }
+    const Frame& fr = frames[f];
     if (tx >= fr.tiles_x || ty >= fr.tiles_y || ((fr.W & 1u) != 0) != ODD) return;
     const uint64_t gt = fr.tile_begin + (uint64_t)ty * fr.tiles_x + tx;
     const uint64_t o = off[gt];
@@ -1347,22 +1371,34 @@ int TiledPlan::launch_bin(const std::vector<Frame>& frames, const std::vector<Dr
 {
     const uint32_t d_lo = frames[f_lo].draw_begin, d_hi = frames[f_hi - 1].draw_end;
     const int tws = log2u(TW), ths = log2u(TH);
+    auto go = [&](dim3 grid, uint32_t d0, const uint2* flat) {
+        auto run = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
+                               d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
+                               d_nl.as<int32_t>(), d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(),
+                               d_careful_in.as<uint4>(), careful_cap, flat);
+        };
+        if (bin_tpt == 1) count_only ? run(k_bin<true, 1>) : run(k_bin<false, 1>);
+        else if (bin_tpt == 2) count_only ? run(k_bin<true, 2>) : run(k_bin<false, 2>);
+        else count_only ? run(k_bin<true, 4>) : run(k_bin<false, 4>);
+        HIP_TRY(hipGetLastError());
+        return SLOTH_OK;
+    };
+    if (flat_bin) {
+        // one block per (draw, BIN_BLOCK * TPT triangles) of the range
+        const int ti = bin_tpt == 1 ? 0 : bin_tpt == 2 ? 1 : 2;
+        const uint64_t b0 = bin_block_begin[ti][d_lo], b1 = bin_block_begin[ti][d_hi];
+        if (b1 - b0 > 0x7FFFFFFFull) return fail(SLOTH_EINVAL, "too many binning blocks in one launch");
+        if (b1 > b0) return go(dim3((unsigned)(b1 - b0)), d_lo, d_bin_blocks[ti].as<uint2>() + b0);
+        return SLOTH_OK;
+    }
+    int rc;
     for (uint32_t d0 = d_lo; d0 < d_hi; d0 += 65535u) {
         const uint32_t d1 = std::min(d_hi, d0 + 65535u);
         uint32_t max_n = 0;
         for (uint32_t d = d0; d < d1; ++d) max_n = std::max(max_n, draws[d].n);
         if (max_n == 0) continue;
-        dim3 grid(blocks(max_n, BIN_BLOCK * (uint32_t)bin_tpt), d1 - d0);
-        auto go = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, grid, dim3(BIN_BLOCK), 0, s, d_draws, d0, d_frames, tws, ths,
-                               d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
-                               d_nl.as<int32_t>(), d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(),
-                               d_careful_in.as<uint4>(), careful_cap);
-        };
-        if (bin_tpt == 1) count_only ? go(k_bin<true, 1>) : go(k_bin<false, 1>);
-        else if (bin_tpt == 2) count_only ? go(k_bin<true, 2>) : go(k_bin<false, 2>);
-        else count_only ? go(k_bin<true, 4>) : go(k_bin<false, 4>);
-        HIP_TRY(hipGetLastError());
+        if ((rc = go(dim3(blocks(max_n, BIN_BLOCK * (uint32_t)bin_tpt), d1 - d0), d0, nullptr))) return rc;
     }
     return SLOTH_OK;
 }
@@ -1394,6 +1430,41 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     if (frames.empty()) return SLOTH_OK;
     const uint32_t F = (uint32_t)frames.size();
     int rc;
+    // 0. flat-grid tables for draws of different sizes / frames of different
+    // tile grids: each k_bin block's (draw, first triangle) for TPT 1, 2, 4,
+    // and each tile's frame
+    flat_bin = false;
+    for (const Draw& d : draws) flat_bin = flat_bin || d.n != draws[0].n;
+    if (flat_bin) {
+        for (int ti = 0; ti < 3; ++ti) {
+            const uint32_t per = (uint32_t)BIN_BLOCK << ti;
+            std::vector<uint2> tbl;
+            std::vector<uint64_t>& beg = bin_block_begin[ti];
+            beg.assign(draws.size() + 1, 0);
+            for (size_t d = 0; d < draws.size(); ++d) {
+                beg[d] = tbl.size();
+                for (uint32_t b = 0; b < draws[d].n; b += per) tbl.push_back(make_uint2((uint32_t)d, b));
+            }
+            beg[draws.size()] = tbl.size();
+            if ((rc = d_bin_blocks[ti].reserve(sizeof(uint2) * std::max<size_t>(1, tbl.size())))) return rc;
+            if (!tbl.empty())
+                HIP_TRY(hipMemcpyAsync(d_bin_blocks[ti].p, tbl.data(), sizeof(uint2) * tbl.size(),
+                                       hipMemcpyHostToDevice, s));
+            HIP_TRY(hipStreamSynchronize(s));   // before tbl goes
+        }
+    }
+    flat_raster = false;
+    for (const Frame& f : frames)
+        flat_raster = flat_raster || f.tiles_x != frames[0].tiles_x || f.tiles_y != frames[0].tiles_y;
+    if (flat_raster) {
+        std::vector<uint32_t> tf(n_tiles);
+        for (uint32_t f = 0; f < F; ++f)
+            for (uint64_t t = 0; t < (uint64_t)frames[f].tiles_x * frames[f].tiles_y; ++t) tf[frames[f].tile_begin + t] = f;
+        if ((rc = d_tile_frame.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, n_tiles)))) return rc;
+        if (n_tiles)
+            HIP_TRY(hipMemcpyAsync(d_tile_frame.p, tf.data(), sizeof(uint32_t) * n_tiles, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
     // 1. count tile entries of CLS_BOX triangles and the careful triangles
     if ((rc = launch_clear(frames, 0, F, s))) return rc;
     if ((rc = launch_bin(frames, draws, d_frames, d_draws, 0, F, true, s))) return rc;
@@ -1460,25 +1531,32 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
             one = one && frames[f].draw_end - frames[f].draw_begin == 1;
             ((frames[f].W & 1u) ? od : ev) = true;
         }
-        // one launch per width parity in the range; each skips the other's frames
+        // one launch per width parity in the range; each skips the other's
+        // frames.  The grid: (tile x, tile y, frame) over the largest tile
+        // grid, or one workgroup per tile when the frames' tile grids differ
+        const uint64_t t_lo = frames[f0].tile_begin;
+        const uint64_t t_hi = frames[f1 - 1].tile_begin + (uint64_t)frames[f1 - 1].tiles_x * frames[f1 - 1].tiles_y;
+        if (flat_raster && t_hi > 0x7FFFFFFFull) return fail(SLOTH_EINVAL, "too many tiles for a flat raster grid");
+        const dim3 grid = flat_raster ? dim3((unsigned)(t_hi - t_lo)) : dim3(mx, my, f1 - f0);
+        const uint32_t* tf = flat_raster ? d_tile_frame.as<uint32_t>() : nullptr;
         if (ev) {
             const auto kern = walk() ? (one ? k_tile_raster<true, true, false> : k_tile_raster<true, false, false>)
                                      : (one ? k_tile_raster<false, true, false> : k_tile_raster<false, false, false>);
-            hipLaunchKernelGGL(kern, dim3(mx, my, f1 - f0), dim3(RASTER_BLOCK), raster_lds(false), s, d_draws,
+            hipLaunchKernelGGL(kern, grid, dim3(RASTER_BLOCK), raster_lds(false), s, d_draws,
                                d_ramps, d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
                                d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
                                d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps, (uint4*)nullptr,
-                               careful_reset);
+                               careful_reset, tf, (uint32_t)t_lo);
             HIP_TRY(hipGetLastError());
             careful_reset = nullptr;
         }
         if (od) {
             const auto kern = one ? k_tile_raster<false, true, true> : k_tile_raster<false, false, true>;
-            hipLaunchKernelGGL(kern, dim3(mx, my, f1 - f0), dim3(RASTER_BLOCK), raster_lds(true), s, d_draws,
+            hipLaunchKernelGGL(kern, grid, dim3(RASTER_BLOCK), raster_lds(true), s, d_draws,
                                d_ramps, d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
                                d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
                                d_careful_box.as<uint4>(), cells, z, win, win_raw ? 1u : 0u, n_ramps,
-                               d_edges.as<uint4>(), careful_reset);
+                               d_edges.as<uint4>(), careful_reset, tf, (uint32_t)t_lo);
             HIP_TRY(hipGetLastError());
             careful_reset = nullptr;
             uint32_t mh = 0;
