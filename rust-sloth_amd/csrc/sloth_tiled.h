@@ -86,6 +86,10 @@ struct TiledPlan {
     // one triangle in 512 or more) for the raster variant whose wavefronts
     // walk those boxes together; SLOTH_BIG_WALK=0/1 overrides
     bool big_walk = false;
+    // set by size(): k_bin reduces newline rows through a segment tree
+    // (one triangle in 256 or more spans more than 16 rows); SLOTH_BIN_TREE
+    // =0/1 overrides
+    bool bin_tree = false;
     // -1: the raster variant big_walk picks; 0 / 1: without / with the
     // wavefront walk of big boxes (set by the autotuner)
     int walk_force = -1;

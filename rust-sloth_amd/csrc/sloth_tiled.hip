@@ -16,6 +16,10 @@ constexpr int RASTER_BLOCK = 256;
 // k_bin counts the triangles whose (unclipped) box has more.
 constexpr uint32_t BIG_BOX = 128;
 constexpr uint32_t BIG_CNT = 2 + MAX_GROUPS;
+// careful_cnt[TALL_CNT]: triangles the counting run finds spanning more than
+// TALL_ROWS rows (TiledPlan::bin_tree)
+constexpr uint32_t TALL_CNT = BIG_CNT + 1;
+constexpr int TALL_ROWS = 16;
 
 // ---------------------------------------------------------------------------
 // k_tile_clear
@@ -29,7 +33,7 @@ __global__ void k_tile_clear(uint32_t* __restrict__ cursor, uint64_t n_tiles, in
     for (uint64_t i = i0; i < n_tiles; i += stride) cursor[i] = 0;
     for (uint64_t i = i0; i < n_rows; i += stride) nl[i] = -1;
     for (uint64_t i = i0; i < n_frames; i += stride) overflow[i] = 0;
-    if (i0 == 0) careful_cnt[0] = careful_cnt[1] = careful_cnt[BIG_CNT] = 0;   // list length, first group's start
+    if (i0 == 0) careful_cnt[0] = careful_cnt[1] = careful_cnt[BIG_CNT] = careful_cnt[TALL_CNT] = 0;   // list length, first group's start
 }
 
 // ---------------------------------------------------------------------------
@@ -209,7 +213,31 @@ __device__ __forceinline__ int wave_max(int v)
 // most 65535 cells a side, tiles of at least 8 x 4 cells).
 constexpr uint32_t GEO_WIDE = 1u << 28, GEO_TALL = 1u << 29, GEO_LOOSE = 1u << 30;
 
-template <bool COUNT, int TPT>
+// Newline rows over a window of BIN_BLOCK rows: a max segment tree in LDS
+// (node 1 spans the window, node i has children 2i and 2i + 1, row r's leaf
+// is BIN_BLOCK + r).  A triangle raises the nodes that tile its rows [a, b)
+// of the window, at most 2 log2(BIN_BLOCK) of them (a range of up to 4 rows
+// takes its leaves); row r's value is the maximum on its leaf's path to the
+// root.
+__device__ __forceinline__ void nl_raise(int32_t* tree, int a, int b, int32_t v)
+{
+    if (b - a <= 4) {
+        for (int y = a; y < b; ++y) atomicMax(&tree[BIN_BLOCK + y], v);
+        return;
+    }
+    for (int l = a + BIN_BLOCK, r = b + BIN_BLOCK; l < r; l >>= 1, r >>= 1) {
+        if (l & 1) atomicMax(&tree[l++], v);
+        if (r & 1) atomicMax(&tree[--r], v);
+    }
+}
+__device__ __forceinline__ int32_t nl_value(const int32_t* tree, int r)
+{
+    int32_t v = -1;
+    for (int i = BIN_BLOCK + r; i >= 1; i >>= 1) v = max(v, tree[i]);
+    return v;
+}
+
+template <bool COUNT, int TPT, bool TREE>
 __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draws, uint32_t d_lo,
                                                    const Frame* __restrict__ frames, int tws, int ths,
                                                    uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
@@ -221,7 +249,9 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
     __shared__ uint32_t w_cnt[BIN_WIN * BIN_WIN];
     __shared__ uint64_t w_base[BIN_WIN * BIN_WIN];   // pool index of the tile's reserved run
     __shared__ uint64_t w_end[BIN_WIN * BIN_WIN];    // end of the tile's pool range
-    __shared__ int32_t s_nl[BIN_BLOCK];
+    // newline rows: a window of rows, or (TREE) its segment tree (nl_raise)
+    __shared__ int32_t s_nl[TREE ? 2 * BIN_BLOCK : BIN_BLOCK];
+    constexpr int LEAF = TREE ? BIN_BLOCK : 0;   // row r of the window at s_nl[LEAF + r]
     __shared__ int s_tx0, s_ty0, s_lo, s_hi;
 
     // the block's draw and first triangle: from the grid, or (flat grids of
@@ -245,6 +275,7 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
 
     w_cnt[threadIdx.x] = 0;
     s_nl[threadIdx.x] = -1;
+    if (TREE) s_nl[LEAF + threadIdx.x] = -1;
     if (threadIdx.x == 0) {
         s_tx0 = s_ty0 = s_lo = 1 << 30;
         s_hi = -1;
@@ -332,6 +363,10 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
             acc_lo = min(acc_lo, t.y0);
             acc_hi = max(acc_hi, t.y1d - 1);
         }
+        if (COUNT) {
+            const uint64_t m = __ballot(has_rows && t.y1d - t.y0 > TALL_ROWS);
+            if (__lane_id() == 0 && m) atomicAdd(&careful_cnt[TALL_CNT], (uint32_t)__popcll(m));
+        }
     }
     __syncthreads();   // LDS initialised
 
@@ -382,7 +417,9 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         }
         if (nl_lds && rows[k]) {
             const int y0 = (int)(rows[k] & 0xFFFFu), y1d = (int)(rows[k] >> 16);
-            for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)(tri0 + (uint32_t)k * BIN_BLOCK));
+            if (TREE) nl_raise(s_nl, y0 - lo, y1d - lo, (int32_t)(tri0 + (uint32_t)k * BIN_BLOCK));
+            else
+                for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - lo], (int32_t)(tri0 + (uint32_t)k * BIN_BLOCK));
         }
     }
     __syncthreads();
@@ -400,9 +437,9 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
             }
         }
     }
-    if (nl_lds) {
-        const int32_t v = s_nl[threadIdx.x];
-        if ((int)threadIdx.x <= hi - lo && v >= 0) atomicMax(&fnl[lo + (int)threadIdx.x], v);
+    if (nl_lds && (int)threadIdx.x <= hi - lo) {
+        const int32_t v = TREE ? nl_value(s_nl, (int)threadIdx.x) : s_nl[threadIdx.x];
+        if (v >= 0) atomicMax(&fnl[lo + (int)threadIdx.x], v);
     }
     __syncthreads();
 
@@ -460,17 +497,23 @@ __global__ void __launch_bounds__(BIN_BLOCK) k_bin(const Draw* __restrict__ draw
         for (int ws = lo; ws <= hi; ws += BIN_BLOCK) {
             __syncthreads();   // the previous window's reads are done
             s_nl[threadIdx.x] = -1;
+            if (TREE) s_nl[LEAF + threadIdx.x] = -1;
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < TPT; ++k) {
                 if (!rows[k]) continue;
-                const int y0 = (int)(rows[k] & 0xFFFFu), y1d = (int)(rows[k] >> 16);
-                for (int y = max(y0, ws); y < min(y1d, ws + BIN_BLOCK); ++y)
-                    atomicMax(&s_nl[y - ws], (int32_t)(tri0 + (uint32_t)k * BIN_BLOCK));
+                const int y0 = max((int)(rows[k] & 0xFFFFu), ws), y1d = min((int)(rows[k] >> 16), ws + BIN_BLOCK);
+                if (TREE) {
+                    if (y0 < y1d) nl_raise(s_nl, y0 - ws, y1d - ws, (int32_t)(tri0 + (uint32_t)k * BIN_BLOCK));
+                } else {
+                    for (int y = y0; y < y1d; ++y) atomicMax(&s_nl[y - ws], (int32_t)(tri0 + (uint32_t)k * BIN_BLOCK));
+                }
             }
             __syncthreads();
-            const int32_t v = s_nl[threadIdx.x];
-            if (ws + (int)threadIdx.x <= hi && v >= 0) atomicMax(&fnl[ws + (int)threadIdx.x], v);
+            if (ws + (int)threadIdx.x <= hi) {
+                const int32_t v = TREE ? nl_value(s_nl, (int)threadIdx.x) : s_nl[threadIdx.x];
+                if (v >= 0) atomicMax(&fnl[ws + (int)threadIdx.x], v);
+            }
         }
     }
 }
@@ -1346,7 +1389,7 @@ int TiledPlan::plan(std::vector<Frame>& frames, uint64_t rows)
     if ((rc = d_off.reserve(sizeof(uint64_t) * (n_tiles + 1)))) return rc;
     if ((rc = d_nl.reserve(sizeof(int32_t) * std::max<uint64_t>(1, n_rows)))) return rc;
     if ((rc = d_overflow.reserve(sizeof(uint32_t) * std::max<size_t>(1, frames.size())))) return rc;
-    if ((rc = d_careful_cnt.reserve(sizeof(uint32_t) * (BIG_CNT + 1)))) return rc;
+    if ((rc = d_careful_cnt.reserve(sizeof(uint32_t) * (TALL_CNT + 1)))) return rc;
     if (has_odd && (rc = d_edges.reserve(sizeof(uint4) * 2 * std::max<uint64_t>(1, n_tiles) * TH))) return rc;
     return SLOTH_OK;
 }
@@ -1378,9 +1421,13 @@ int TiledPlan::launch_bin(const std::vector<Frame>& frames, const std::vector<Dr
                                d_nl.as<int32_t>(), d_overflow.as<uint32_t>(), d_careful_cnt.as<uint32_t>(),
                                d_careful_in.as<uint4>(), careful_cap, flat);
         };
-        if (bin_tpt == 1) count_only ? run(k_bin<true, 1>) : run(k_bin<false, 1>);
-        else if (bin_tpt == 2) count_only ? run(k_bin<true, 2>) : run(k_bin<false, 2>);
-        else count_only ? run(k_bin<true, 4>) : run(k_bin<false, 4>);
+        // the counting run takes the tree variant (it counts the tall triangles)
+        if (bin_tpt == 1)
+            count_only ? run(k_bin<true, 1, true>) : bin_tree ? run(k_bin<false, 1, true>) : run(k_bin<false, 1, false>);
+        else if (bin_tpt == 2)
+            count_only ? run(k_bin<true, 2, true>) : bin_tree ? run(k_bin<false, 2, true>) : run(k_bin<false, 2, false>);
+        else
+            count_only ? run(k_bin<true, 4, true>) : bin_tree ? run(k_bin<false, 4, true>) : run(k_bin<false, 4, false>);
         HIP_TRY(hipGetLastError());
         return SLOTH_OK;
     };
@@ -1468,7 +1515,7 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     // 1. count tile entries of CLS_BOX triangles and the careful triangles
     if ((rc = launch_clear(frames, 0, F, s))) return rc;
     if ((rc = launch_bin(frames, draws, d_frames, d_draws, 0, F, true, s))) return rc;
-    uint32_t cnt[BIG_CNT + 1];
+    uint32_t cnt[TALL_CNT + 1];
     HIP_TRY(hipMemcpyAsync(cnt, d_careful_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     const uint32_t n_careful = cnt[0];
@@ -1478,6 +1525,14 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     for (const Draw& d : draws) n_tri += d.n;
     big_walk = (uint64_t)cnt[BIG_CNT] * 512 >= n_tri && cnt[BIG_CNT] > 0;
     if (const char* e = getenv("SLOTH_BIG_WALK")) big_walk = e[0] == '1';
+    // the binning kernel's segment-tree newline rows pay once one triangle
+    // in 256 spans more than TALL_ROWS rows (a wavefront's row loop lasts as
+    // long as its tallest triangle's)
+    const bool tree = (uint64_t)cnt[TALL_CNT] * 256 >= n_tri && cnt[TALL_CNT] > 0;
+    bool want_tree = tree;
+    if (const char* e = getenv("SLOTH_BIN_TREE")) want_tree = e[0] == '1';
+    if (want_tree != bin_tree) ++generation;   // captured graphs hold the other variant
+    bin_tree = want_tree;
     if (n_careful > careful_cap) {
         // 2. room for the careful list, then count again so that it is filled
         careful_cap = n_careful + n_careful / 4 + 256;

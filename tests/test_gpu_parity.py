@@ -74,13 +74,17 @@ def test_random_soups(sloth, W, H):
     _check(sloth, f"random soups {W}x{H}", W, H, U, draws)
 
 
+@pytest.mark.parametrize("bin_tree", ["auto", "0", "1"])
 @pytest.mark.parametrize("W,H", [(301, 600), (300, 600)])
-def test_tall_frames_newline_rows(sloth, W, H):
+def test_tall_frames_newline_rows(sloth, W, H, bin_tree, monkeypatch):
     """Frames taller than a block of 256 triangles can span: a UV sphere
     (each block's triangles share a narrow band of rows, away from row 0),
     scattered small triangles and a few that cover most of the frame (a
     block's rows span more than 256).  The newline rows and the frame's
-    cells must match the oracle on each path."""
+    cells must match the oracle on each path, with the binning kernel's
+    newline rows per row or through its segment tree (SLOTH_BIN_TREE)."""
+    if bin_tree != "auto":
+        monkeypatch.setenv("SLOTH_BIN_TREE", bin_tree)
     rng = np.random.default_rng(W + H)
     from sloth_amd import meshgen
 
