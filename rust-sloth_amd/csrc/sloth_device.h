@@ -65,7 +65,7 @@ struct Tri {
     float x1, y1, z1, x2, y2, z2, x3, y3, z3;
     float A0, B0, A1, B1, A2, B2;   // edge deltas of orient(v2,v3,.), orient(v3,v1,.), orient(v1,v2,.)
     float a;                        // 1 / orient(v1,v2,v3)      rasterizer.rs:67
-    float minx, maxx, miny, maxy;   // the vertices' extent
+    float minx, maxx;               // the vertices' x extent
     int x0, x1d, y0, y1d;           // pixel domain [x0,x1d) x [y0,y1d)  rasterizer.rs:58-66
 };
 
@@ -125,12 +125,12 @@ __device__ __forceinline__ void tri_finish(uint32_t W, uint32_t H, Tri& t)
 {
     t.minx = fminf(t.x1, fminf(t.x2, t.x3));
     t.maxx = fmaxf(t.x1, fmaxf(t.x2, t.x3));
-    t.miny = fminf(t.y1, fminf(t.y2, t.y3));
-    t.maxy = fmaxf(t.y1, fmaxf(t.y2, t.y3));
+    const float miny = fminf(t.y1, fminf(t.y2, t.y3));
+    const float maxy = fmaxf(t.y1, fmaxf(t.y2, t.y3));
     t.x0 = sat_index(ceilf(fmaxf(t.minx, 1.0f)));
-    t.y0 = sat_index(ceilf(fmaxf(t.miny, 1.0f)));
+    t.y0 = sat_index(ceilf(fmaxf(miny, 1.0f)));
     t.x1d = sat_index(ceilf(fminf(t.maxx * 2.0f, (float)(W - 1))));
-    t.y1d = sat_index(ceilf(fminf(t.maxy, (float)(H - 1))));
+    t.y1d = sat_index(ceilf(fminf(maxy, (float)(H - 1))));
     t.A0 = t.x3 - t.x2;
     t.B0 = t.y3 - t.y2;
     t.A1 = t.x1 - t.x3;
@@ -298,11 +298,13 @@ enum : uint32_t { CLS_EMPTY = 0, CLS_BOX = 1, CLS_SPANS = 2, CLS_FULL = 3, CLS_T
 // DX = max(|X1 - minx|, |X0 - maxx|) (|X0 - x| <= X1 - minx when x <= X0,
 // <= maxx - X0 otherwise; likewise for X1), so the sum is at most
 // (sum |A_k|) DY + (sum |B_k|) DX.  Its few roundings (relative error 2^-24
-// each) are within the margins of the bounds that use it.
-__device__ __forceinline__ float edge_terms_bound(const Tri& t, float X0, float X1, float Y0, float Y1)
+// each) are within the margins of the bounds that use it.  miny, maxy: the
+// vertices' y extent.
+__device__ __forceinline__ float edge_terms_bound(const Tri& t, float X0, float X1, float Y0, float Y1, float miny,
+                                                  float maxy)
 {
     const float DX = fmaxf(fabsf(X1 - t.minx), fabsf(X0 - t.maxx));
-    const float DY = fmaxf(fabsf(Y1 - t.miny), fabsf(Y0 - t.maxy));
+    const float DY = fmaxf(fabsf(Y1 - miny), fabsf(Y0 - maxy));
     return (fabsf(t.A0) + fabsf(t.A1) + fabsf(t.A2)) * DY + (fabsf(t.B0) + fabsf(t.B1) + fabsf(t.B2)) * DX;
 }
 
@@ -344,7 +346,8 @@ __device__ __forceinline__ uint32_t setup_class(const Tri& t)
     const float K = 8.0f * 5.9604645e-08f;
     // a non-empty domain has X0 <= X1 and Y0 <= Y1 (an empty one is CLS_EMPTY)
     const float X0 = (float)t.x0, X1 = (float)(t.x1d - 1), Y0 = (float)t.y0, Y1 = (float)(t.y1d - 1);
-    const float bound = K * edge_terms_bound(t, X0, X1, Y0, Y1);
+    const float bound = K * edge_terms_bound(t, X0, X1, Y0, Y1, fminf(t.y1, fminf(t.y2, t.y3)),
+                                             fmaxf(t.y1, fmaxf(t.y2, t.y3)));
     float ey = t.y3 - t.y1, ex = t.x3 - t.x1;
     float o = t.A2 * ey - t.B2 * ex;
     float eo = K * (fabsf(t.A2) * fabsf(ey) + fabsf(t.B2) * fabsf(ex));

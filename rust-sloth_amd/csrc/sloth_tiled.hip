@@ -787,7 +787,11 @@ __device__ __forceinline__ uint32_t entry_tri(uint32_t e, const uint4* __restric
 // is finished by k_tile_edges, from the winners this tile and its left
 // neighbour leave in `edge_out` (first and last column, per row).
 // Tile (tx, ty) of frame fr, global tile index gt, whose list is pool[o, o + n).
-template <bool BIG, bool ONE, bool ODD>
+// FLAT is that of the calling kernel and changes nothing here: it gives each
+// kernel its own copy of the lambdas below, which the compiler then inlines
+// as it does for a single caller (shared by the two kernels, they cost the
+// 3-D instances two VGPRs spilled to scratch).
+template <bool BIG, bool ONE, bool ODD, bool FLAT>
 __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, const Ramp* __restrict__ ramps,
                                             const Ramp* s_ramp, const Frame& fr, uint32_t tx, uint32_t ty, uint64_t gt,
                                             uint64_t o, uint32_t n, int tws, int ths, const uint32_t* __restrict__ pool,
@@ -945,9 +949,8 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         // sum to the exact orient, the two additions add at most 2.0001u of
         // their inputs' magnitudes, and the FP32 orient o errs from the exact
         // one by at most 4.0002u (|A2| |y3 - y1| + |B2| |x3 - x1|).  So every
-        // S lies within E = 10u (Sb + Q) of o, with Sb a bound of the edge
-        // terms' sum over the box (edge_terms_bound) and Q o's terms; 10u
-        // also covers the rounding of E
+        // S lies within E = 10u (Sb + Q) of o, with Sb the edge terms' sum
+        // over the box and Q o's terms; 10u also covers the rounding of E
         // and of o -+ E (Sb >= |o|).  fl(na * .) and the step index are
         // monotone, so equal indices at the two ends mean one glyph for the
         // whole box.  Shaders that are not non-decreasing, and non-finite
@@ -963,8 +966,15 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
         // it is at most thr[i] (thresholds are non-decreasing).
         auto box_glyph = [&](const Ramp& rp, const Tri& t, float na, int bx0, int bx1, int by0, int by1) -> uint32_t {
             if (!rp.sorted) return ~0u;
-            // (an empty box has no pixels, so its glyph is never used)
-            const float Sb = edge_terms_bound(t, (float)bx0, (float)bx1, (float)by0, (float)by1);
+            // the edge terms' exact maximum over the box, edge by edge (an
+            // empty box has no pixels, so its glyph is never used); the
+            // cheaper edge_terms_bound would also do, but its schedule costs
+            // this kernel a register spill
+            const float X0 = (float)bx0, X1 = (float)bx1, Y0 = (float)by0, Y1 = (float)by1;
+            const float Sb =
+                fabsf(t.A0) * fmaxf(fabsf(Y0 - t.y2), fabsf(Y1 - t.y2)) + fabsf(t.B0) * fmaxf(fabsf(X0 - t.x2), fabsf(X1 - t.x2)) +
+                fabsf(t.A1) * fmaxf(fabsf(Y0 - t.y3), fabsf(Y1 - t.y3)) + fabsf(t.B1) * fmaxf(fabsf(X0 - t.x3), fabsf(X1 - t.x3)) +
+                fabsf(t.A2) * fmaxf(fabsf(Y0 - t.y1), fabsf(Y1 - t.y1)) + fabsf(t.B2) * fmaxf(fabsf(X0 - t.x1), fabsf(X1 - t.x1));
             const float ey = t.y3 - t.y1, ex = t.x3 - t.x1;
             const float o = t.A2 * ey - t.B2 * ex;
             const float E = 5.9604645e-07f * (Sb + fabsf(t.A2) * fabsf(ey) + fabsf(t.B2) * fabsf(ex));
@@ -1231,10 +1241,12 @@ __device__ __forceinline__ void raster_tile(const Draw* __restrict__ draws, cons
 // smaller frame's tile grid return at once.
 // Occupancy: LDS allows 7 workgroups of 4 wavefronts per CU.  Left to itself
 // the compiler gives the instances 76-80 VGPRs (6 wavefronts per SIMD); held
-// to 7 they fit 72 with SGPR spills to VGPR lanes only, and run ~3% faster.
-// The big-box instance for several draws per frame would spill VGPRs to
-// scratch at 72 and keeps 6.
-template <bool BIG, bool ONE, bool ODD>
+// to 7 they fit 72 and run ~3% faster.  Most fit with SGPR spills to VGPR
+// lanes only; the big-box instance for one draw per frame and the plain one
+// for several spill 2 VGPRs to scratch, still faster than at 6 (a batch of
+// uneven single-draw frames: 0.559 against 0.587 ms).  The big-box instance
+// for several draws per frame would spill more at 72 and keeps 6.
+template <bool BIG, bool ONE, bool ODD, bool FLAT>
 __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_eu(BIG && !ONE ? 6 : 7, 8))) k_tile_raster(
     const Draw* __restrict__ draws, const Ramp* __restrict__ ramps, const Frame* __restrict__ frames, uint32_t f_lo,
     int tws, int ths, uint32_t* __restrict__ cursor, const uint64_t* __restrict__ off,
@@ -1250,10 +1262,10 @@ __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_
 
     // the careful list was read by k_careful, which ran before this launch
     if (careful_reset && (blockIdx.x | blockIdx.y | blockIdx.z | threadIdx.x) == 0) careful_reset[0] = 0;
-    // the block's tile: (x, y, frame) from a 3-D grid, or (frames of
+    // the block's tile: (x, y, frame) from a 3-D grid, or (FLAT: frames of
     // different tile grids) tile t_lo + x of a flat one
     uint32_t f, tx, ty;
-    if (tile_frame) {
+    if constexpr (FLAT) {
         const uint32_t g = t_lo + blockIdx.x;
         f = tile_frame[g];
         const uint32_t local = g - (uint32_t)frames[f].tile_begin;
@@ -1273,7 +1285,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_
     if (n > 0)
         for (uint32_t i = threadIdx.x; i < min(n_ramps, LDS_RAMPS) * (uint32_t)sizeof(Ramp) / 4; i += blockDim.x)
             ((uint32_t*)s_ramp)[i] = ((const uint32_t*)ramps)[i];
-    raster_tile<BIG, ONE, ODD>(draws, ramps, s_ramp, fr, tx, ty, gt, o, n, tws, ths, pool, nl, careful_box, cells, zbuf,
+    raster_tile<BIG, ONE, ODD, FLAT>(draws, ramps, s_ramp, fr, tx, ty, gt, o, n, tws, ths, pool, nl, careful_box, cells, zbuf,
                                win, win_raw, edge_out, smem);
     // The tile leaves its binning state as k_tile_clear would for the next
     // render: its cursor at 0 and, for even widths (k_tile_edges reads odd
@@ -1563,6 +1575,20 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     return SLOTH_OK;
 }
 
+// The raster instance for a width parity: big-box walk or not, one draw per
+// frame or not, flat grid (frames of different tile grids) or 3-D.  Each
+// instance is compiled on its own, so the 3-D instances carry no flat-grid
+// mapping and its registers.
+template <bool ODD, bool FLAT>
+static auto raster_kernel(bool walk, bool one)
+{
+    if constexpr (ODD)
+        return one ? k_tile_raster<false, true, true, FLAT> : k_tile_raster<false, false, true, FLAT>;
+    else
+        return walk ? (one ? k_tile_raster<true, true, false, FLAT> : k_tile_raster<true, false, false, FLAT>)
+                    : (one ? k_tile_raster<false, true, false, FLAT> : k_tile_raster<false, false, false, FLAT>);
+}
+
 int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_frames, const Draw* d_draws,
                              const Ramp* d_ramps, uint32_t f_lo, uint32_t f_hi, uint32_t* cells, float* z, int32_t* win,
                              bool win_raw, uint32_t n_ramps, bool reset_careful, hipStream_t s)
@@ -1595,8 +1621,7 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
         const dim3 grid = flat_raster ? dim3((unsigned)(t_hi - t_lo)) : dim3(mx, my, f1 - f0);
         const uint32_t* tf = flat_raster ? d_tile_frame.as<uint32_t>() : nullptr;
         if (ev) {
-            const auto kern = walk() ? (one ? k_tile_raster<true, true, false> : k_tile_raster<true, false, false>)
-                                     : (one ? k_tile_raster<false, true, false> : k_tile_raster<false, false, false>);
+            const auto kern = flat_raster ? raster_kernel<false, true>(walk(), one) : raster_kernel<false, false>(walk(), one);
             hipLaunchKernelGGL(kern, grid, dim3(RASTER_BLOCK), raster_lds(false), s, d_draws,
                                d_ramps, d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
                                d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
@@ -1606,7 +1631,7 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
             careful_reset = nullptr;
         }
         if (od) {
-            const auto kern = one ? k_tile_raster<false, true, true> : k_tile_raster<false, false, true>;
+            const auto kern = flat_raster ? raster_kernel<true, true>(false, one) : raster_kernel<true, false>(false, one);
             hipLaunchKernelGGL(kern, grid, dim3(RASTER_BLOCK), raster_lds(true), s, d_draws,
                                d_ramps, d_frames, f0, log2u(TW), log2u(TH), d_cursor.as<uint32_t>(),
                                d_off.as<uint64_t>(), d_pool.as<uint32_t>(), d_nl.as<int32_t>(),
