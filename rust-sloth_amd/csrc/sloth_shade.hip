@@ -190,12 +190,12 @@ __global__ void __launch_bounds__(SHADE_BLOCK) k_shade_fwd(ShadeArgs p)
     block_partials(loss, 0.0, 0.0, 0.0, p.partial, blockIdx.x);
 }
 
-constexpr uint32_t BWD_TILE_W = 64, BWD_TILE_H = 32;   // z entries per backward tile
+constexpr uint32_t BWD_TILE_W = 64, BWD_TILE_H = 16;   // z entries per backward tile
 constexpr uint32_t BWD_TABLE = 256;     // LDS rows for the tile's distinct vertices
 constexpr uint32_t BWD_PROBES = 32;     // a vertex that finds no row adds to HBM directly
 constexpr uint64_t BWD_EMPTY = ~0ull;
-constexpr uint32_t BWD_RUN = 8;         // consecutive z entries of a row per thread
-constexpr uint32_t BWD_GROUP = 4;       // run entries whose loads are issued together
+constexpr uint32_t BWD_RUN = 4;         // consecutive z entries of a row per thread
+constexpr uint32_t BWD_GROUP = 2;       // run entries whose loads are issued together
 constexpr uint32_t BWD_ROW = 7;         // 64-bit LDS words per vertex row (6 terms, padded)
 static_assert(BWD_TILE_W % BWD_RUN == 0 && SHADE_BLOCK / (BWD_TILE_W / BWD_RUN) == BWD_TILE_H,
               "the block's runs cover one tile in one pass");
@@ -206,7 +206,7 @@ static_assert(BWD_TABLE == SHADE_BLOCK, "one thread per table row at the end of 
 // claimed (the claiming sums' largest binary exponent + FIX_HEADROOM), so
 // that |integer| < 2^FIX_BITS for every |t| < 2^e and 2^(63 - FIX_BITS)
 // adds cannot overflow (a row takes at most one add per z entry of the
-// tile, 2048).  A term outside the row's range (or not finite) goes to the
+// tile, 1024).  A term outside the row's range (or not finite) goes to the
 // binary64 accumulator directly.  The rounding error of an add is at most
 // 2^(e - FIX_BITS - 1), i.e. 2^-39 of the claiming sums' magnitude.  LDS has
 // integer atomics at full rate, while a float atomic add takes several
@@ -224,7 +224,7 @@ __device__ __forceinline__ int fix_exponent(const float* t)
     return min(e, 100) + FIX_HEADROOM;
 }
 
-// Backward: a block takes 64 x BWD_TILE_H tiles of z entries (tile j of the
+// Backward: a block takes BWD_TILE_W x BWD_TILE_H tiles of z entries (tile j of the
 // job; frames hold tile_begin[f] .. tile_begin[f+1]), grid-striding over the
 // tiles, each thread a run of BWD_RUN entries of a row (every other entry
 // for even widths, whose odd entries hold no winner).  The run's visibility
