@@ -81,7 +81,12 @@ struct TiledPlan {
     bool flat_bin = false, flat_raster = false;
     DevBuf d_bin_blocks[3];                      // per TPT 1, 2, 4: (draw, first triangle) of each block
     std::vector<uint64_t> bin_block_begin[3];    // per TPT: each draw's first block (draws + 1)
-    DevBuf d_tile_frame;                         // each tile's frame
+    // flat raster grids: workgroup i of a launch over tiles [t_lo, t_hi) takes
+    // entry t_lo + i, a (tile, frame) pair: in tile order, or (launches of the
+    // whole batch) with the tiles of more than four chunks of entries first,
+    // so that a tile of thousands of entries (a dense mesh in a small frame)
+    // does not start last
+    DevBuf d_tile_frame, d_tile_heavy;
     // set by size(): the lists hold enough big boxes (more than 128 pixels,
     // one triangle in 512 or more) for the raster variant whose wavefronts
     // walk those boxes together; SLOTH_BIG_WALK=0/1 overrides

@@ -1253,7 +1253,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_
     const uint32_t* __restrict__ pool, int32_t* __restrict__ nl, const uint4* __restrict__ careful_box,
     uint32_t* __restrict__ cells, float* __restrict__ zbuf, int32_t* __restrict__ win, uint32_t win_raw,
     uint32_t n_ramps, uint4* __restrict__ edge_out, uint32_t* __restrict__ careful_reset,
-    const uint32_t* __restrict__ tile_frame, uint32_t t_lo)
+    const uint2* __restrict__ tile_frame, uint32_t t_lo)
 {
     extern __shared__ uint64_t smem[];
     // the job's first LDS_RAMPS shaders, so that a glyph lookup reads LDS
@@ -1263,12 +1263,12 @@ __global__ void __launch_bounds__(RASTER_BLOCK) __attribute__((amdgpu_waves_per_
     // the careful list was read by k_careful, which ran before this launch
     if (careful_reset && (blockIdx.x | blockIdx.y | blockIdx.z | threadIdx.x) == 0) careful_reset[0] = 0;
     // the block's tile: (x, y, frame) from a 3-D grid, or (FLAT: frames of
-    // different tile grids) tile t_lo + x of a flat one
+    // different tile grids) the (tile, frame) of entry t_lo + x of a flat one
     uint32_t f, tx, ty;
     if constexpr (FLAT) {
-        const uint32_t g = t_lo + blockIdx.x;
-        f = tile_frame[g];
-        const uint32_t local = g - (uint32_t)frames[f].tile_begin;
+        const uint2 e = tile_frame[t_lo + blockIdx.x];
+        f = e.y;
+        const uint32_t local = e.x - (uint32_t)frames[f].tile_begin;
         ty = local / frames[f].tiles_x;
         tx = local - ty * frames[f].tiles_x;
     } else {
@@ -1489,9 +1489,8 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     if (frames.empty()) return SLOTH_OK;
     const uint32_t F = (uint32_t)frames.size();
     int rc;
-    // 0. flat-grid tables for draws of different sizes / frames of different
-    // tile grids: each k_bin block's (draw, first triangle) for TPT 1, 2, 4,
-    // and each tile's frame
+    // 0. flat k_bin grids (draws of different sizes): each block's (draw,
+    // first triangle) for TPT 1, 2, 4
     flat_bin = false;
     for (const Draw& d : draws) flat_bin = flat_bin || d.n != draws[0].n;
     if (flat_bin) {
@@ -1511,18 +1510,6 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
                                        hipMemcpyHostToDevice, s));
             HIP_TRY(hipStreamSynchronize(s));   // before tbl goes
         }
-    }
-    flat_raster = false;
-    for (const Frame& f : frames)
-        flat_raster = flat_raster || f.tiles_x != frames[0].tiles_x || f.tiles_y != frames[0].tiles_y;
-    if (flat_raster) {
-        std::vector<uint32_t> tf(n_tiles);
-        for (uint32_t f = 0; f < F; ++f)
-            for (uint64_t t = 0; t < (uint64_t)frames[f].tiles_x * frames[f].tiles_y; ++t) tf[frames[f].tile_begin + t] = f;
-        if ((rc = d_tile_frame.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, n_tiles)))) return rc;
-        if (n_tiles)
-            HIP_TRY(hipMemcpyAsync(d_tile_frame.p, tf.data(), sizeof(uint32_t) * n_tiles, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
     }
     // 1. count tile entries of CLS_BOX triangles and the careful triangles
     if ((rc = launch_clear(frames, 0, F, s))) return rc;
@@ -1567,6 +1554,33 @@ int TiledPlan::size(const std::vector<Frame>& frames, const std::vector<Draw>& d
     }
     if ((rc = d_pool.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, off[n_tiles])))) return rc;
     HIP_TRY(hipMemcpyAsync(d_off.p, off.data(), sizeof(uint64_t) * (n_tiles + 1), hipMemcpyHostToDevice, s));
+    // 4. flat raster grids (frames of different tile grids): each tile's
+    // (tile, frame) in tile order, and the same with the tiles of more than
+    // four chunks of entries moved to the front, longest first (the rest
+    // keep tile order, whose neighbouring tiles share cached vertices and
+    // lists: sorting every tile by length made the uneven batch slower)
+    flat_raster = false;
+    for (const Frame& f : frames)
+        flat_raster = flat_raster || f.tiles_x != frames[0].tiles_x || f.tiles_y != frames[0].tiles_y;
+    std::vector<uint2> tf, heavy;
+    if (flat_raster) {
+        tf.resize(n_tiles);
+        for (uint32_t f = 0; f < F; ++f)
+            for (uint64_t t = 0; t < (uint64_t)frames[f].tiles_x * frames[f].tiles_y; ++t)
+                tf[frames[f].tile_begin + t] = make_uint2((uint32_t)(frames[f].tile_begin + t), f);
+        heavy = tf;
+        const auto split = std::stable_partition(heavy.begin(), heavy.end(),
+                                                 [&](const uint2& a) { return counts[a.x] > 4 * CHUNK; });
+        std::stable_sort(heavy.begin(), split,
+                         [&](const uint2& a, const uint2& b) { return counts[a.x] > counts[b.x]; });
+        if ((rc = d_tile_frame.reserve(sizeof(uint2) * std::max<uint64_t>(1, n_tiles))) ||
+            (rc = d_tile_heavy.reserve(sizeof(uint2) * std::max<uint64_t>(1, n_tiles))))
+            return rc;
+        if (n_tiles) {
+            HIP_TRY(hipMemcpyAsync(d_tile_frame.p, tf.data(), sizeof(uint2) * n_tiles, hipMemcpyHostToDevice, s));
+            HIP_TRY(hipMemcpyAsync(d_tile_heavy.p, heavy.data(), sizeof(uint2) * n_tiles, hipMemcpyHostToDevice, s));
+        }
+    }
     // renders start from cleared cursors, rows and careful list (each leaves them so)
     if ((rc = launch_clear(frames, 0, F, s))) return rc;
     HIP_TRY(hipStreamSynchronize(s));
@@ -1619,7 +1633,9 @@ int TiledPlan::launch_raster(const std::vector<Frame>& frames, const Frame* d_fr
         const uint64_t t_hi = frames[f1 - 1].tile_begin + (uint64_t)frames[f1 - 1].tiles_x * frames[f1 - 1].tiles_y;
         if (flat_raster && t_hi > 0x7FFFFFFFull) return fail(SLOTH_EINVAL, "too many tiles for a flat raster grid");
         const dim3 grid = flat_raster ? dim3((unsigned)(t_hi - t_lo)) : dim3(mx, my, f1 - f0);
-        const uint32_t* tf = flat_raster ? d_tile_frame.as<uint32_t>() : nullptr;
+        const uint2* tf = !flat_raster ? nullptr
+                          : f0 == 0 && f1 == (uint32_t)frames.size() ? d_tile_heavy.as<uint2>()
+                                                                      : d_tile_frame.as<uint2>();
         if (ev) {
             const auto kern = flat_raster ? raster_kernel<false, true>(walk(), one) : raster_kernel<false, false>(walk(), one);
             hipLaunchKernelGGL(kern, grid, dim3(RASTER_BLOCK), raster_lds(false), s, d_draws,

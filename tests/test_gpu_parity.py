@@ -724,3 +724,43 @@ def test_mixed_parity_batch(sloth, tile, raster_path):
             ref = oracle_render(W, H, U, [(m.xyz, m.rgb, R, None)], image=image)
             cells, z, win = b.read_frame(f, want_winners=(rep == 1))
             assert_frame_equal(cells, z, win if rep == 1 else None, ref, W)
+
+
+def test_long_tiles_first(sloth, raster_path):
+    """A batch whose frames differ in tile grid takes a flat raster grid; a
+    dense mesh in a small frame puts thousands of entries (more than four
+    chunks) on one tile, which the whole-batch launch moves to the front of
+    the grid.  Every frame matches the oracle through the graph launch (that
+    order), the per-frame launches (tile order) and the re-render that
+    returns winners."""
+    from sloth_amd import meshgen
+
+    if raster_path == "global":
+        pytest.skip("tile order only exists on the tiled path")
+    dense = meshgen.displaced_icosphere(5, seed=61)    # 20,480 triangles
+    light = meshgen.displaced_icosphere(2, seed=62)
+    sizes = [(160, 48, light), (80, 24, dense), (64, 30, light), (81, 25, dense), (200, 64, dense)]
+    frames, draws, plan = [], [], []
+    for f, (W, H, m) in enumerate(sizes):
+        sm = sloth.SimpleMesh.from_mesh(m)
+        U = meshgen.screen_transform(W, H, [m.bbox_max()])
+        R = meshgen.rotation_euler(0.3 * f, 1.1 + f, 0.2)
+        frames.append(sloth.FrameSpec(W, H, U, True))
+        draws.append(sloth.DrawSpec(f, sm, R))
+        plan.append((W, H, U, R, m, sm))
+    b = sloth.Batch(frames, draws)
+    b.set_tile(64, 32)
+    for mode in ("graph", "per-frame", "graph-again"):
+        say(f"case long tiles first, {mode}")
+        if mode == "per-frame":
+            b.run_per_frame()
+        else:
+            b.run()
+        b.sync()
+        for f, (W, H, U, R, m, _) in enumerate(plan):
+            ref = oracle_render(W, H, U, [(m.xyz, m.rgb, R, None)], image=True)
+            cells, z, _ = b.read_frame(f)
+            assert_frame_equal(cells, z, None, ref, W)
+            if mode == "graph-again":
+                cells, z, win = b.read_frame(f, want_winners=True)
+                assert_frame_equal(cells, z, win, ref, W)
