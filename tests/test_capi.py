@@ -1,6 +1,6 @@
 """CPU-side checks of the C-ABI library: it loads, exports exactly the
-functions include/sloth_hip.h declares, the Python binding declares the same
-set, and argument errors come back as status codes without touching a GPU."""
+functions include/sloth_hip.h declares, the Python binding and the Rust
+binding of INTEGRATION.md declare the same set, and argument errors come back as status codes without touching a GPU."""
 from __future__ import annotations
 
 import ctypes as C
@@ -38,6 +38,16 @@ def test_binding_covers_header():
     import sloth_amd
 
     assert sorted(sloth_amd.SIGNATURES) == declared_functions()
+
+
+def test_rust_binding_covers_header():
+    """The Rust extern block in INTEGRATION.md declares every entry point of
+    the header, and nothing else."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = text[text.index('extern "C" {'):]
+    block = block[:block.index("\n}")]
+    names = sorted(set(re.findall(r"\bpub fn (sloth_[a-z0-9_]+)\s*\(", block)))
+    assert names == declared_functions()
 
 
 def test_struct_layouts():
