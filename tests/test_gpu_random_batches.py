@@ -1,0 +1,89 @@
+"""Seeded random batches against the oracle (GPU).
+
+Each case draws a batch of 1-6 frames of random size (1-320 x 1-100 cells,
+either parity, image or terminal mode) with 0-3 draws each (random soups from
+sub-pixel to frame-filling triangles, slivers, triangles far off screen,
+closed meshes, empty meshes), a batch-wide shader, and a random plan: tile
+size, big-box walk and the binning kernel's triangles per thread.  Every
+frame's cells and depths must equal the oracle's (rasterizer.rs:39-93), and
+the winners of the last frame too; the graph is replayed once and checked
+again.  The cases mix the plan's paths: flat and 3-D raster grids, one and
+several draws per frame, both raster parities."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from _helpers import assert_frame_equal, oracle_render, random_soup, say
+
+pytestmark = pytest.mark.gpu
+
+TILES = [(8, 4), (16, 8), (32, 8), (64, 16), (64, 32), (128, 16), (256, 8)]
+
+
+def _mesh(rng, kind, W, H, U):
+    from sloth_amd import meshgen
+
+    if kind == "soup":
+        return random_soup(rng, int(rng.integers(1, 2000)), W, H, U, size=float(rng.choice([0.01, 0.05, 0.2])))
+    if kind == "big":
+        return random_soup(rng, int(rng.integers(1, 40)), W, H, U, size=1.0)
+    if kind == "far":
+        xyz, rgb = random_soup(rng, int(rng.integers(1, 300)), W, H, U, spread=40.0, size=2.0)
+        return xyz, rgb
+    if kind == "sliver":
+        n = int(rng.integers(1, 400))
+        a = rng.uniform(-1, 1, size=(n, 3))
+        d = rng.normal(0.0, 0.4, size=(n, 3))
+        perp = rng.normal(0.0, 1.0, size=(n, 3)) * rng.choice([1e-3, 1e-5, 0.0], size=(n, 1))
+        xyz = np.stack([a, a + d, a + 0.5 * d + perp], 1).astype(np.float32)
+        return xyz, rng.integers(0, 256, size=(n, 3), dtype=np.uint8)
+    if kind == "empty":
+        return np.zeros((0, 3, 3), np.float32), np.zeros((0, 3), np.uint8)
+    m = meshgen.displaced_icosphere(int(rng.integers(1, 5)), seed=int(rng.integers(0, 1000)))
+    return m.xyz, m.rgb
+
+
+@pytest.mark.parametrize("seed", range(64))
+def test_random_batch(sloth, seed, monkeypatch):
+    from sloth_amd import Shader, meshgen
+
+    rng = np.random.default_rng(1000 + seed)
+    shaders = [None, Shader((-0.1, 0.25, 0.75, 1.0, 2.0), "01234", "#"),
+               Shader((0.9, 0.3, 0.6, 0.3, 0.95), "vwxyz", "q"),
+               Shader(tuple(-0.05 + 0.0625 * k for k in range(16)), "ABCDEFGHIJKLMNOP", "!")]
+    sh = shaders[int(rng.integers(0, len(shaders)))]
+    n_frames = int(rng.integers(1, 7))
+    frames, draws, plan = [], [], []
+    for f in range(n_frames):
+        W, H = int(rng.integers(1, 321)), int(rng.integers(1, 101))
+        image = bool(rng.integers(0, 2))
+        U = meshgen.screen_transform(W, H, [np.array([1.0, 1.0, 1.0], np.float32)])
+        frames.append(sloth.FrameSpec(W, H, U, image))
+        fd = []
+        for _ in range(int(rng.integers(0, 4))):
+            kind = str(rng.choice(["soup", "soup", "big", "far", "sliver", "closed", "empty"]))
+            xyz, rgb = _mesh(rng, kind, W, H, U)
+            R = meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3))
+            draws.append(sloth.DrawSpec(f, sloth.SimpleMesh(xyz, rgb), R))
+            fd.append((xyz, rgb, R, sh))
+        plan.append((W, H, U, image, fd))
+    tile = TILES[int(rng.integers(0, len(TILES)))]
+    walk = int(rng.integers(0, 2))
+    tpt = int(rng.choice([1, 2, 4]))
+    say(f"case random batch seed {seed}: {n_frames} frames {[(p[0], p[1], p[3], len(p[4])) for p in plan]}, "
+        f"tile {tile}, walk {walk}, {tpt} per thread, shader {sh is not None}")
+    b = sloth.Batch(frames, draws, sh) if sh is not None else sloth.Batch(frames, draws)
+    b.set_tile(*tile)
+    b.set_walk(walk)
+    b.set_bin(tpt)
+    for rep in range(2):
+        b.run()
+        b.sync()
+        for f, (W, H, U, image, fd) in enumerate(plan):
+            ref = oracle_render(W, H, U, fd, image=image)
+            want = rep == 1 and f == n_frames - 1
+            cells, z, win = b.read_frame(f, want_winners=want)
+            assert_frame_equal(cells, z, win if want else None, ref, W)
