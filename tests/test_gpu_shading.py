@@ -62,7 +62,7 @@ def test_visibility_matches_oracle(sloth):
     assert sh.n_slots == 3 * 96 * 64
 
 
-# 100 x 70 and 101 x 45: backward tiles and 8-entry runs cut by the frame's
+# 100 x 70 and 101 x 45: backward tiles and 4-entry runs cut by the frame's
 # right and bottom edges (101 is odd: its visibility comes from the odd-width
 # tile raster)
 @pytest.mark.parametrize("views,W,H", [(4, 128, 96), (2, 256, 256), (3, 100, 70), (2, 101, 45)])
@@ -216,3 +216,49 @@ def test_shading_on_indexed_mesh(sloth):
     # run: equal to the last few binary32 bits
     for a, b in zip(s0[1:], s1[1:]):
         assert np.linalg.norm(a.astype(np.float64) - b) <= 1e-6 * np.linalg.norm(a.astype(np.float64))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_views_match_fp64_adjoint(sloth, seed):
+    """Seeded random shading jobs: frame sizes from 1 x 1 up (odd and even
+    widths, widths below one 4-entry run, a single row), 1-3 views, spheres
+    of random resolution.  Loss and gradients against the FP64 adjoint; a
+    job whose views show no triangle has zero loss and gradients."""
+    from sloth_amd import meshgen
+
+    rng = np.random.default_rng(500 + seed)
+    W = int(rng.choice([1, 2, 3, 5, 7, 31, 64, 97, 130]))
+    H = int(rng.choice([1, 2, 3, 17, 40, 75]))
+    views = int(rng.integers(1, 4))
+    n_lon, n_lat = int(rng.integers(6, 40)), int(rng.integers(4, 24))
+    m, U, Rs, batch, vc, vn, L, rng2 = setup_views(sloth, n_lon, n_lat, views, W, H, seed=600 + seed)
+    say(f"case random shading seed {seed}: uv {n_lon}x{n_lat} ({m.n} tris), {views} views {W}x{H}")
+    sh = sloth.Shading(batch, m.faces, m.verts.shape[0])
+    vc_t = (vc + rng2.normal(0, 0.1, vc.shape)).astype(np.float32)
+    L_t = (rot_y(10.0) @ L.astype(np.float64)).astype(np.float32)
+    _, target = sh.forward(vc_t, vn, L_t, want_out=True)
+    sh.set_target(target)
+    loss_s, dc, dn, dl = sh.step(vc, vn, L)
+
+    frames = oracle_frames(m, U, Rs, W, H)
+    loss_cpu = 0.0
+    gc = np.zeros_like(vc, dtype=np.float64)
+    gn = np.zeros_like(vn, dtype=np.float64)
+    gl = np.zeros(3)
+    nz = W * H
+    for f, (R, fr) in enumerate(zip(Rs, frames)):
+        M = oracle.mat4_mul(U, R)
+        lo, _, (a, b, c) = oracle.shade_f64(fr, M, m.xyz, m.faces, vc, vn, L, target[f * nz:(f + 1) * nz],
+                                           want_out=False, want_grad=True)
+        loss_cpu += lo
+        gc += a
+        gn += b
+        gl += c
+    assert abs(loss_s - loss_cpu) <= REL * abs(loss_cpu) + 1e-30
+    for name, g, ref in (("color", dc, gc), ("normal", dn, gn), ("light", dl, gl)):
+        nref = np.linalg.norm(ref)
+        if nref == 0.0:
+            assert np.linalg.norm(g.astype(np.float64)) == 0.0, name
+            continue
+        err = np.linalg.norm(g.astype(np.float64) - ref) / nref
+        assert err <= REL, (name, err)
