@@ -8,7 +8,8 @@ size, big-box walk and the binning kernel's triangles per thread.  Every
 frame's cells and depths must equal the oracle's (rasterizer.rs:39-93), and
 the winners of the last frame too; the graph is replayed once and checked
 again.  The cases mix the plan's paths: flat and 3-D raster grids, one and
-several draws per frame, both raster parities."""
+several draws per frame, both raster parities; each case also runs on the
+general global-key path."""
 from __future__ import annotations
 
 import math
@@ -46,9 +47,16 @@ def _mesh(rng, kind, W, H, U):
     return m.xyz, m.rgb
 
 
+@pytest.mark.parametrize("path", ["tiled", "global"])
 @pytest.mark.parametrize("seed", range(64))
-def test_random_batch(sloth, seed, monkeypatch):
+def test_random_batch(sloth, seed, path, monkeypatch):
     from sloth_amd import Shader, meshgen
+
+    # the general global-key path (SLOTH_FORCE_GLOBAL=1) has no tile plan
+    if path == "global":
+        monkeypatch.setenv("SLOTH_FORCE_GLOBAL", "1")
+    else:
+        monkeypatch.delenv("SLOTH_FORCE_GLOBAL", raising=False)
 
     rng = np.random.default_rng(1000 + seed)
     shaders = [None, Shader((-0.1, 0.25, 0.75, 1.0, 2.0), "01234", "#"),
@@ -73,12 +81,13 @@ def test_random_batch(sloth, seed, monkeypatch):
     tile = TILES[int(rng.integers(0, len(TILES)))]
     walk = int(rng.integers(0, 2))
     tpt = int(rng.choice([1, 2, 4]))
-    say(f"case random batch seed {seed}: {n_frames} frames {[(p[0], p[1], p[3], len(p[4])) for p in plan]}, "
+    say(f"case random batch seed {seed} [{path}]: {n_frames} frames {[(p[0], p[1], p[3], len(p[4])) for p in plan]}, "
         f"tile {tile}, walk {walk}, {tpt} per thread, shader {sh is not None}")
     b = sloth.Batch(frames, draws, sh) if sh is not None else sloth.Batch(frames, draws)
-    b.set_tile(*tile)
-    b.set_walk(walk)
-    b.set_bin(tpt)
+    if path == "tiled":
+        b.set_tile(*tile)
+        b.set_walk(walk)
+        b.set_bin(tpt)
     for rep in range(2):
         b.run()
         b.sync()
