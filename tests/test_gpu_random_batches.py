@@ -7,7 +7,7 @@ closed meshes, empty meshes), a batch-wide shader, and a random plan: tile
 size, big-box walk and the binning kernel's triangles per thread.  Every
 frame's cells and depths must equal the oracle's (rasterizer.rs:39-93), and
 the winners of the last frame too; the graph is replayed once and checked
-again.  The cases mix the plan's paths: flat and 3-D raster grids, one and
+again, with each frame's encoded byte streams (plain, colour, HTML).  The cases mix the plan's paths: flat and 3-D raster grids, one and
 several draws per frame, both raster parities; each case also runs on the
 general global-key path."""
 from __future__ import annotations
@@ -17,6 +17,7 @@ import math
 import numpy as np
 import pytest
 
+import oracle
 from _helpers import assert_frame_equal, oracle_render, random_soup, say
 
 pytestmark = pytest.mark.gpu
@@ -96,3 +97,7 @@ def test_random_batch(sloth, seed, path, monkeypatch):
             want = rep == 1 and f == n_frames - 1
             cells, z, win = b.read_frame(f, want_winners=want)
             assert_frame_equal(cells, z, win if want else None, ref, W)
+            if rep == 1:
+                # Context::flush's byte streams (context.rs:50-92) of the frame
+                for mode in (sloth.ENCODE_TEXT, sloth.ENCODE_ANSI, sloth.ENCODE_HTML):
+                    assert b.encode_frame(f, mode) == oracle.encode(ref.cells, mode, image), (f, mode)
