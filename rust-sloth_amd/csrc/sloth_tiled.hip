@@ -1467,8 +1467,10 @@ int TiledPlan::launch_careful(const Frame* d_frames, const Draw* d_draws, bool c
 {
     if (careful_cap == 0) return SLOTH_OK;
     const int tws = log2u(TW), ths = log2u(TH);
-    // a fixed grid of wavefronts strides over the careful list (one triangle per wavefront at a time)
-    dim3 grid(std::min(blocks(careful_cap, 4), 1024u));
+    // a grid of up to 16,384 wavefronts (the chip holds about 8,192 of
+    // these at once) strides over the careful list, one triangle per
+    // wavefront at a time (the sweep's pass 0.015 -> 0.012 ms against 4,096)
+    dim3 grid(std::min(blocks(careful_cap, 4), 4096u));
     if (count_only)
         hipLaunchKernelGGL(k_careful<true>, grid, dim3(256), 0, s, d_frames, tws, ths,
                            d_cursor.as<uint32_t>(), d_off.as<uint64_t>(), d_pool.as<uint32_t>(),
