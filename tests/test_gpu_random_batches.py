@@ -101,3 +101,53 @@ def test_random_batch(sloth, seed, path, monkeypatch):
                 # Context::flush's byte streams (context.rs:50-92) of the frame
                 for mode in (sloth.ENCODE_TEXT, sloth.ENCODE_ANSI, sloth.ENCODE_HTML):
                     assert b.encode_frame(f, mode) == oracle.encode(ref.cells, mode, image), (f, mode)
+
+
+def _random_shader(rng):
+    from sloth_amd import Shader
+
+    kind = int(rng.integers(0, 5))
+    n = int(rng.integers(0, 17))
+    thr = np.sort(rng.uniform(-0.2, 1.2, n)) if kind < 3 else rng.uniform(-0.2, 1.2, n)
+    thr = [float(t) for t in thr]
+    if kind == 1 and n > 1:
+        thr[-1] = float("inf")
+    if kind == 2 and n > 2:
+        thr[1] = thr[0]
+    glyphs = "".join(chr(int(c)) for c in rng.integers(33, 127, n))
+    return Shader(tuple(thr), glyphs, chr(int(rng.integers(33, 127))))
+
+
+@pytest.mark.parametrize("path", ["tiled", "global"])
+@pytest.mark.parametrize("seed", range(32))
+def test_random_context_shaders(sloth, seed, path, monkeypatch):
+    """One frame through the Context API (draw_mesh per draw,
+    rasterizer.rs:39-46) with a random shader per draw: sorted and unsorted
+    step tables, repeated and infinite thresholds, from no step to all 16;
+    cells, depths and winners against the oracle."""
+    from sloth_amd import meshgen
+
+    if path == "global":
+        monkeypatch.setenv("SLOTH_FORCE_GLOBAL", "1")
+    else:
+        monkeypatch.delenv("SLOTH_FORCE_GLOBAL", raising=False)
+    rng = np.random.default_rng(3000 + seed)
+    W, H = int(rng.integers(1, 200)), int(rng.integers(1, 70))
+    image = bool(rng.integers(0, 2))
+    U = meshgen.screen_transform(W, H, [np.array([1.0, 1.0, 1.0], np.float32)])
+    ctx = sloth.Context(W, H, image=image)
+    ctx.utransform = U
+    ctx.clear()
+    draws, keep = [], []
+    for _ in range(int(rng.integers(1, 6))):
+        kind = str(rng.choice(["soup", "big", "sliver", "closed"]))
+        xyz, rgb = _mesh(rng, kind, W, H, U)
+        R = meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3))
+        sh = _random_shader(rng)
+        m = sloth.SimpleMesh(xyz, rgb)
+        keep.append(m)
+        sloth.draw_mesh(ctx, m, R, sh)
+        draws.append((xyz, rgb, R, sh))
+    say(f"case random context seed {seed} [{path}]: {W}x{H} image={image} draws={len(draws)}")
+    cells, z, win = ctx.read(want_winners=True)
+    assert_frame_equal(cells, z, win, oracle_render(W, H, U, draws, image=image), W)
