@@ -93,3 +93,42 @@ def test_noalloc_matches_context_path(sloth):
         assert e.value.code == sloth.SLOTH_EINVAL
     finally:
         dev.free()
+
+
+def test_noalloc_random_sequence(sloth):
+    """Twenty calls in a row on one device, each frame of a random size (up to
+    2560 x 1440, the largest that fits the fixed scratch, and down to one
+    cell), parity and mode, with random soups or closed meshes: each call
+    starts from what the previous one left in the shared scratch, and each
+    frame must equal the oracle's."""
+    from sloth_amd import meshgen
+
+    rng = np.random.default_rng(77)
+    dev = _Dev(sloth)
+    try:
+        for k in range(20):
+            if k == 7:
+                W, H = 2560, 1440
+            else:
+                W, H = int(rng.integers(1, 400)), int(rng.integers(1, 300))
+            image = bool(rng.integers(0, 2))
+            if rng.integers(0, 2):
+                m = meshgen.displaced_icosphere(int(rng.integers(1, 5)), seed=int(rng.integers(0, 100)))
+                xyz, rgb = m.xyz, m.rgb
+            else:
+                n = int(rng.integers(1, 3000))
+                xyz = (rng.uniform(-1, 1, (n, 1, 3)) + rng.normal(0, 0.1, (n, 3, 3))).astype(np.float32)
+                rgb = rng.integers(0, 256, (n, 3), dtype=np.uint8)
+            U = meshgen.screen_transform(W, H, [np.array([1.0, 1.0, 1.0], np.float32)])
+            R = meshgen.rotation_euler(*rng.uniform(0, 6.283, 3))
+            say(f"case noalloc sequence {k}: {W}x{H} image={image} {len(xyz)} tris")
+            nc = W * H + (H if image else 0)
+            d_xyz, d_rgb = dev.upload(xyz), dev.upload(rgb)
+            d_cells, d_z = dev.alloc(4 * nc), dev.alloc(4 * W * H)
+            sloth.render_frame_noalloc(d_xyz, d_rgb, len(xyz), W, H, image, U, R, d_cells, d_z)
+            cells = dev.download(d_cells, nc, np.uint32)
+            z = dev.download(d_z, W * H, np.float32)
+            assert_frame_equal(cells, z, None, oracle_render(W, H, U, [(xyz, rgb, R, None)], image=image), W)
+            dev.free()
+    finally:
+        dev.free()
