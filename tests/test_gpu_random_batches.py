@@ -4,7 +4,8 @@ Each case draws a batch of 1-6 frames of random size (1-320 x 1-100 cells,
 either parity, image or terminal mode) with 0-3 draws each (random soups from
 sub-pixel to frame-filling triangles, slivers, triangles far off screen,
 closed meshes, empty meshes), a batch-wide shader, and a random plan: tile
-size, big-box walk and the binning kernel's triangles per thread.  Every
+size, big-box walk and the binning kernel's triangles per thread; half the
+closed meshes are indexed device meshes.  Every
 frame's cells and depths must equal the oracle's (rasterizer.rs:39-93), and
 the winners of the last frame too; the graph is replayed once and checked
 again, with each frame's encoded byte streams (plain, colour, HTML).  The cases mix the plan's paths: flat and 3-D raster grids, one and
@@ -45,7 +46,7 @@ def _mesh(rng, kind, W, H, U):
     if kind == "empty":
         return np.zeros((0, 3, 3), np.float32), np.zeros((0, 3), np.uint8)
     m = meshgen.displaced_icosphere(int(rng.integers(1, 5)), seed=int(rng.integers(0, 1000)))
-    return m.xyz, m.rgb
+    return m.xyz, m.rgb, m
 
 
 @pytest.mark.parametrize("path", ["tiled", "global"])
@@ -74,9 +75,12 @@ def test_random_batch(sloth, seed, path, monkeypatch):
         fd = []
         for _ in range(int(rng.integers(0, 4))):
             kind = str(rng.choice(["soup", "soup", "big", "far", "sliver", "closed", "empty"]))
-            xyz, rgb = _mesh(rng, kind, W, H, U)
+            xyz, rgb, *closed = _mesh(rng, kind, W, H, U)
             R = meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3))
-            draws.append(sloth.DrawSpec(f, sloth.SimpleMesh(xyz, rgb), R))
+            # closed meshes: a soup or (half of them) an indexed device mesh
+            sm = sloth.SimpleMesh.from_mesh(closed[0], indexed=True) if closed and rng.integers(0, 2) else \
+                sloth.SimpleMesh(xyz, rgb)
+            draws.append(sloth.DrawSpec(f, sm, R))
             fd.append((xyz, rgb, R, sh))
         plan.append((W, H, U, image, fd))
     tile = TILES[int(rng.integers(0, len(TILES)))]
@@ -141,7 +145,7 @@ def test_random_context_shaders(sloth, seed, path, monkeypatch):
     draws, keep = [], []
     for _ in range(int(rng.integers(1, 6))):
         kind = str(rng.choice(["soup", "big", "sliver", "closed"]))
-        xyz, rgb = _mesh(rng, kind, W, H, U)
+        xyz, rgb, *_ = _mesh(rng, kind, W, H, U)
         R = meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3))
         sh = _random_shader(rng)
         m = sloth.SimpleMesh(xyz, rgb)
