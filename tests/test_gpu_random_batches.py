@@ -8,7 +8,10 @@ size, big-box walk and the binning kernel's triangles per thread; half the
 closed meshes are indexed device meshes.  Every
 frame's cells and depths must equal the oracle's (rasterizer.rs:39-93), and
 the winners of the last frame too; the graph is replayed once and checked
-again, with each frame's encoded byte streams (plain, colour, HTML).  The cases mix the plan's paths: flat and 3-D raster grids, one and
+again, with each frame's encoded byte streams (plain, colour, HTML); then
+every draw takes a new transform (some zoomed, so that tile lists can
+outgrow their sizing and the batch re-plans) and every frame is checked once
+more.  The cases mix the plan's paths: flat and 3-D raster grids, one and
 several draws per frame, both raster parities; each case also runs on the
 general global-key path."""
 from __future__ import annotations
@@ -105,6 +108,25 @@ def test_random_batch(sloth, seed, path, monkeypatch):
                 # Context::flush's byte streams (context.rs:50-92) of the frame
                 for mode in (sloth.ENCODE_TEXT, sloth.ENCODE_ANSI, sloth.ENCODE_HTML):
                     assert b.encode_frame(f, mode) == oracle.encode(ref.cells, mode, image), (f, mode)
+    # new transforms for every draw (some zoomed, so that tile lists can
+    # outgrow their sizing and the batch re-plans), then render and check again
+    if draws:
+        news = []
+        for fd in (p[4] for p in plan):
+            for j, (xyz, rgb, R, dsh) in enumerate(fd):
+                R2 = np.asarray(meshgen.rotation_euler(*rng.uniform(0, 2 * math.pi, 3)), np.float32).reshape(4, 4).copy()
+                if rng.integers(0, 2):
+                    R2[:3, :3] *= np.float32(rng.uniform(1.0, 3.0))
+                fd[j] = (xyz, rgb, R2.reshape(-1), dsh)
+        # the order of the draw array: draws were appended frame by frame
+        for fd in (p[4] for p in plan):
+            news.extend(d[2] for d in fd)
+        b.set_transforms(np.stack(news))
+        b.run()
+        b.sync()
+        for f, (W, H, U, image, fd) in enumerate(plan):
+            cells, z, _ = b.read_frame(f)
+            assert_frame_equal(cells, z, None, oracle_render(W, H, U, fd, image=image), W)
 
 
 def _random_shader(rng):
