@@ -112,6 +112,8 @@ def main() -> int:
     ap.add_argument("--walk", type=int, default=None, help="0/1: the tile raster variant without/with the "
                     "wavefront walk of big boxes (with --tile)")
     ap.add_argument("--shading-only", action="store_true", help="only the shading leg (for profilers)")
+    ap.add_argument("--uneven-only", default=None, metavar="WxH,WALK,TPT",
+                    help="only the uneven-batch leg, at this plan instead of the autotuner's (for profilers)")
     ap.add_argument("--graph-only", action="store_true",
                     help="only the batched graph renders (for profilers): no plain, stage, CPU or other legs")
     ap.add_argument("--no-extra", action="store_true", help="skip the 4K, uneven-batch and shading legs")
@@ -133,6 +135,10 @@ def main() -> int:
     if args.shading_only:
         sloth_amd.init(0)
         print(json.dumps({"fwd_bwd": leg_shading(sloth_amd, meshgen, args)}), flush=True)
+        return 0
+    if args.uneven_only:
+        sloth_amd.init(0)
+        print(json.dumps({"uneven_batch": leg_uneven(sloth_amd, meshgen, args)}), flush=True)
         return 0
 
     F = args.frames
@@ -342,16 +348,23 @@ def leg_uneven(sloth_amd, meshgen, args):
         tris += m.n
         alg += 4 * (W * H + H) + 4 * W * H + 39 * m.n
     b = sloth_amd.Batch(frames, draws)
-    if not args.no_tune:
+    if args.uneven_only:
+        tile, walk, tpt = args.uneven_only.lower().split(",")
+        b.set_tile(*(int(v) for v in tile.split("x")))
+        b.set_walk(int(walk))
+        b.set_bin(int(tpt))
+    elif not args.no_tune:
         b.autotune()
     ms = statistics.median(b.time(5, 2) / 5 for _ in range(5))
-    pms = b.time(1, 1, per_frame=True)
+    # (profiling runs keep to the batched launches)
+    pms = None if args.uneven_only else b.time(1, 1, per_frame=True)
     gbs, frac = _frac(alg, ms)
     return {"workload": f"64 meshes, {tris} triangles total (1k..1M log-uniform, seed 3), resolutions from "
                         f"{sorted(set((f.width, f.height) for f in frames))}",
             "plan": b.plan(), "ms_per_batch": round(ms, 4), "meshes_per_s": round(64e3 / ms, 1),
-            "gtri_per_s": round(tris / ms / 1e6, 3), "plain_ms": round(pms, 4),
-            "batched_over_plain": round(pms / ms, 2), "alg_bytes": alg, "achieved_GBps": gbs, "frac": frac}
+            "gtri_per_s": round(tris / ms / 1e6, 3), "plain_ms": round(pms, 4) if pms else None,
+            "batched_over_plain": round(pms / ms, 2) if pms else None, "alg_bytes": alg, "achieved_GBps": gbs,
+            "frac": frac}
 
 
 def leg_shading(sloth_amd, meshgen, args):
