@@ -112,6 +112,8 @@ def main() -> int:
     ap.add_argument("--walk", type=int, default=None, help="0/1: the tile raster variant without/with the "
                     "wavefront walk of big boxes (with --tile)")
     ap.add_argument("--shading-only", action="store_true", help="only the shading leg (for profilers)")
+    ap.add_argument("--frame4k-only", default=None, metavar="WxH,WALK,TPT",
+                    help="only the 4K-frame leg, at this plan instead of the autotuner's (for profilers)")
     ap.add_argument("--uneven-only", default=None, metavar="WxH,WALK,TPT",
                     help="only the uneven-batch leg, at this plan instead of the autotuner's (for profilers)")
     ap.add_argument("--graph-only", action="store_true",
@@ -135,6 +137,10 @@ def main() -> int:
     if args.shading_only:
         sloth_amd.init(0)
         print(json.dumps({"fwd_bwd": leg_shading(sloth_amd, meshgen, args)}), flush=True)
+        return 0
+    if args.frame4k_only:
+        sloth_amd.init(0)
+        print(json.dumps({"frame_4k": leg_4k(sloth_amd, meshgen, args)}), flush=True)
         return 0
     if args.uneven_only:
         sloth_amd.init(0)
@@ -320,7 +326,12 @@ def leg_4k(sloth_amd, meshgen, args):
     m, W, H, U, R = meshgen.workload_icosphere_4k()
     mesh = sloth_amd.SimpleMesh.from_mesh(m)
     b = sloth_amd.Batch([sloth_amd.FrameSpec(W, H, U, True)], [sloth_amd.DrawSpec(0, mesh, R)])
-    if not args.no_tune:
+    if args.frame4k_only:
+        tile, walk, tpt = args.frame4k_only.lower().split(",")
+        b.set_tile(*(int(v) for v in tile.split("x")))
+        b.set_walk(int(walk))
+        b.set_bin(int(tpt))
+    elif not args.no_tune:
         b.autotune()
     ms = statistics.median(b.time(10, 2, per_frame=True) / 10 for _ in range(5))
     msg = statistics.median(b.time(10, 2) / 10 for _ in range(5))
